@@ -1,0 +1,133 @@
+/*
+ * structured_gpu.h -- C ABI of the MI355X residual / tangent / adjoint library.
+ *
+ * Plain pointers and sizes only.  Host-buffer entry points take and return the
+ * reference's array-of-structures layout q[i][j][k] (i over nic = ni-1 cells,
+ * j over njc = nj-1 cells, k over the nvar unknowns per cell, k fastest;
+ * src/solver/solution.cpp:28).  Device-buffer entry points (suffix _dev) take
+ * device pointers in the library's struct-of-arrays layout:
+ * a[k*plane + j*ldi + i] with ldi = sg_ldi(ctx) and plane = njc*ldi.
+ *
+ * Every function returns 0 (SG_OK) or a negative SG_ERR_* code; the text of
+ * the last error of a context is available from sg_last_error().  One context
+ * owns one HIP stream and its device buffers; a context is not thread-safe.
+ * Host-buffer calls are synchronous; _dev calls are ordered on the context's
+ * stream and are synchronous as well unless the name ends in _async.
+ */
+#ifndef STRUCTURED_GPU_H
+#define STRUCTURED_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SG_OK 0
+#define SG_ERR_ARG (-1)     /* bad argument or shape                         */
+#define SG_ERR_BC (-2)      /* boundary list reads a ghost before writing it */
+#define SG_ERR_NAN (-3)     /* non-finite residual / norm                    */
+#define SG_ERR_HIP (-4)     /* HIP runtime error                             */
+#define SG_ERR_NOCONV (-5)  /* linear or nonlinear solve did not converge    */
+#define SG_ERR_IO (-6)      /* file missing, short or malformed             */
+#define SG_ERR_STATE (-7)   /* call out of order (e.g. no state uploaded)    */
+
+/* Boundary types and faces, src/model/bc.h:8-11 and bc.cpp:465-526. */
+enum sg_bc_type {
+  SG_BC_FREESTREAM = 0,
+  SG_BC_OUTFLOW = 1,
+  SG_BC_SLIPWALL = 2,
+  SG_BC_WALL = 3,        /* adiabatic no-slip wall ("wall")            */
+  SG_BC_ISOTHERMAL = 4,  /* "isothermalwall"                           */
+  SG_BC_PERIODIC = 5,
+  SG_BC_WAKE = 6
+};
+enum sg_face { SG_FACE_BOTTOM = 0, SG_FACE_RIGHT = 1, SG_FACE_TOP = 2, SG_FACE_LEFT = 3 };
+
+/* [freestream] table, src/utils/config.cpp:31-43. */
+typedef struct sg_freestream {
+  double rho_inf, u_inf, v_inf, p_inf, T_inf;
+  double mu_inf;   /* > 1e-15 switches the viscous terms on                 */
+  double pr_inf;
+  double aoa_deg;  /* used only to rotate the wall forces into Cl / Cd      */
+  double nut_inf;  /* freestream nu~ for the SA model (absolute value)      */
+} sg_freestream;
+
+/* One [[boundary]] entry.  start/end are ghosted indices along the face,
+ * inclusive; a negative end counts from the far end (end -> n+2+end),
+ * src/model/bc.cpp:435-457. */
+typedef struct sg_bc {
+  int type;
+  int face;
+  long long start;
+  long long end;
+  double u, v, T;
+} sg_bc;
+
+typedef struct sg_opts {
+  int order;       /* MUSCL order, 1 or 2 (solver.order)                    */
+  int turbulence;  /* 0: laminar (4 unknowns), 1: Spalart-Allmaras (5)      */
+  int tail;        /* geometry.tail: surface cells are [tail-1, ni-tail)    */
+  int reserved;
+  double dpdx, dpdy;  /* [source] body force                                */
+} sg_opts;
+
+typedef struct sg_ctx sg_ctx;
+
+/* ---- setup: Mesh::setup + EulerEquation ctor + BoundaryContainer,
+ *      src/utils/mesh.cpp:391-414, src/model/eulerequation.cpp:31-133 ---- */
+
+/* xv, yv: vertex coordinates [ni][nj] (nj fastest, as Mesh::xv). */
+int sg_create(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs,
+              const sg_bc* bcs, int nbc, const sg_opts* opts, sg_ctx** out);
+void sg_destroy(sg_ctx* ctx);
+const char* sg_last_error(const sg_ctx* ctx);
+int sg_nvar(const sg_ctx* ctx);
+int sg_ldi(const sg_ctx* ctx);
+/* Per-cell SA production multiplier beta [nic][njc] (default 1). */
+int sg_set_beta(sg_ctx* ctx, const double* beta);
+/* Wall distance [nic][njc] used by the SA destruction term.  sg_create
+ * computes it from the wall boundaries; this overrides it. */
+int sg_set_walldist(sg_ctx* ctx, const double* d);
+int sg_get_walldist(sg_ctx* ctx, double* d);
+/* Metric arrays as the reference's Mesh holds them:
+ * normal_chi [ni][njc][2], normal_eta [nic][nj][2], volume/xc/yc [nic][njc]. */
+int sg_get_metrics(sg_ctx* ctx, double* normal_chi, double* normal_eta, double* volume, double* xc, double* yc);
+
+/* ---- residual: EulerEquation::calc_residual, src/model/eulerequation.cpp:202-232
+ *      (R = dq/dt, already divided by the cell volume), plus the per-equation
+ *      L2 norm of Solver::step, src/solver/solver.cpp:125-134.  l2 may be NULL. */
+int sg_residual(sg_ctx* ctx, const double* q_aos, double* r_aos, double* l2);
+
+/* ---- tangent: J.v with J = dR/dq at q (replaces the ADOL-C tape +
+ *      sparse_jac forward sweep, src/solver/solver.cpp:69-90,153-158). */
+int sg_jv(sg_ctx* ctx, const double* q_aos, const double* v_aos, double* jv_aos);
+
+/* ---- adjoint: J^T.psi and (dR/dbeta)^T.psi (no counterpart in the
+ *      reference, which has no reverse mode; dbeta is [nic][njc]). */
+int sg_jtv(sg_ctx* ctx, const double* q_aos, const double* psi_aos, double* jtpsi_aos, double* dbeta);
+
+/* ---- local time step: EulerEquation::calc_dt, src/model/eulerequation.cpp:237-259 */
+int sg_dt(sg_ctx* ctx, const double* q_aos, double cfl, double* dt_aos);
+
+/* ---- wall quantities: IOManager::write_surface, src/utils/io.cpp:181-255.
+ *      x, cp, cf have nb = ni-2*tail+1 entries; forces = {cl_p, cd_p, cl_v,
+ *      cd_v, cl, cd}.  Evaluates the residual at q internally, because the
+ *      wall Cf uses the eta-face gradients of that evaluation. */
+int sg_wall(sg_ctx* ctx, const double* q_aos, double* x, double* cp, double* cf, double* forces);
+
+/* ---- device-resident variants (SoA device pointers, see header comment) */
+int sg_residual_dev(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_host);
+int sg_jv_dev(sg_ctx* ctx, const double* q_dev, const double* v_dev, double* jv_dev);
+int sg_jtv_dev(sg_ctx* ctx, const double* q_dev, const double* psi_dev, double* jtpsi_dev, double* dbeta_dev);
+/* Enqueue the residual only (no norm readback, no synchronisation). */
+int sg_residual_dev_async(sg_ctx* ctx, const double* q_dev, double* r_dev);
+int sg_synchronize(sg_ctx* ctx);
+/* Layout conversion between the reference AoS and the library SoA. */
+int sg_aos_to_soa_dev(sg_ctx* ctx, const double* aos_dev, double* soa_dev, int nvar);
+int sg_soa_to_aos_dev(sg_ctx* ctx, const double* soa_dev, double* aos_dev, int nvar);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STRUCTURED_GPU_H */

@@ -1,0 +1,193 @@
+// CPU ORACLE — test infrastructure only (see sg_oracle.hpp).  extern "C"
+// wrappers so that tests/ and the benchmark's CPU-baseline leg can call the
+// restatement through ctypes.  It shares only the plain-data structs of
+// include/structured_gpu.h with the product.
+#include <omp.h>
+#include <chrono>
+#include <memory>
+
+#include "../include/structured_gpu.h"
+#include "sg_oracle.hpp"
+
+using namespace oracle;
+
+namespace {
+
+struct Setup {
+  Geometry geo;
+  Params prm;
+  std::vector<Bc> bcs;
+};
+
+long resolve_end(long end, int face, size_t nic, size_t njc) {
+  if (end >= 0) return end;
+  return (face == FACE_BOTTOM || face == FACE_TOP) ? (long)nic + 2 + end : (long)njc + 2 + end;
+}
+
+void make_setup(Setup& s, int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs,
+                const sg_bc* bcs, int nbc, const sg_opts* o) {
+  s.geo.build((size_t)ni, (size_t)nj, xv, yv);
+  s.prm.rho_inf = fs->rho_inf; s.prm.u_inf = fs->u_inf; s.prm.v_inf = fs->v_inf;
+  s.prm.p_inf = fs->p_inf; s.prm.T_inf = fs->T_inf; s.prm.mu_inf = fs->mu_inf;
+  s.prm.pr_inf = fs->pr_inf; s.prm.aoa = fs->aoa_deg * M_PI / 180.0; s.prm.nut_inf = fs->nut_inf;
+  s.prm.dpdx = o->dpdx; s.prm.dpdy = o->dpdy; s.prm.order = o->order; s.prm.sa = o->turbulence ? 1 : 0;
+  s.bcs.clear();
+  for (int b = 0; b < nbc; ++b) {
+    Bc c;
+    c.type = bcs[b].type; c.face = bcs[b].face; c.start = (long)bcs[b].start;
+    c.end = resolve_end((long)bcs[b].end, c.face, s.geo.nic, s.geo.njc);
+    c.u = bcs[b].u; c.v = bcs[b].v; c.T = bcs[b].T;
+    s.bcs.push_back(c);
+  }
+}
+
+template <class T>
+void init_res(Residual<T>& r, const Setup& s, const double* wd, const double* beta) {
+  r.init(&s.geo, s.prm, s.bcs);
+  const size_t n = s.geo.nic * s.geo.njc;
+  if (wd) r.wall_dist.assign(wd, wd + n);
+  else { r.wall_dist.assign(n, 1.0); if (s.prm.sa) wall_distance(s.geo, s.bcs, r.wall_dist.data()); }
+  if (beta) r.beta.assign(beta, beta + n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int og_metrics(int ni, int nj, const double* xv, const double* yv, double* nchi, double* neta, double* vol,
+               double* xc, double* yc, double* ds_chi, double* ds_eta) {
+  Geometry g;
+  g.build((size_t)ni, (size_t)nj, xv, yv);
+  std::copy(g.normal_chi.a.begin(), g.normal_chi.a.end(), nchi);
+  std::copy(g.normal_eta.a.begin(), g.normal_eta.a.end(), neta);
+  std::copy(g.volume.a.begin(), g.volume.a.end(), vol);
+  std::copy(g.xc.a.begin(), g.xc.a.end(), xc);
+  std::copy(g.yc.a.begin(), g.yc.a.end(), yc);
+  std::copy(g.ds_chi.a.begin(), g.ds_chi.a.end(), ds_chi);
+  std::copy(g.ds_eta.a.begin(), g.ds_eta.a.end(), ds_eta);
+  return 0;
+}
+
+// R(q) in the reference layout; l2 (nvar entries) may be NULL.  wd / beta may
+// be NULL (computed wall distance, beta = 1).
+int og_residual(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs,
+                int nbc, const sg_opts* o, const double* wd, const double* beta, const double* q, double* r,
+                double* l2) {
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  Residual<double> res;
+  init_res(res, s, wd, beta);
+  res.evaluate(q, r);
+  if (l2) l2norm(s.geo.nic * s.geo.njc, (size_t)s.prm.nvar(), r, l2);
+  return 0;
+}
+
+// J.v by one forward dual sweep of the same residual.
+int og_jv(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs, int nbc,
+          const sg_opts* o, const double* wd, const double* beta, const double* q, const double* v, double* r,
+          double* jv) {
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  Residual<Dual<1>> res;
+  init_res(res, s, wd, beta);
+  const size_t n = s.geo.nic * s.geo.njc * (size_t)s.prm.nvar();
+  std::vector<Dual<1>> qd(n), rd(n);
+  for (size_t m = 0; m < n; ++m) { qd[m].v = q[m]; qd[m].d[0] = v[m]; }
+  res.evaluate(qd.data(), rd.data());
+  for (size_t m = 0; m < n; ++m) { if (r) r[m] = rd[m].v; jv[m] = rd[m].d[0]; }
+  return 0;
+}
+
+// dR/dbeta . w (the beta-direction derivative), by a dual in the beta field.
+int og_dbeta(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs, int nbc,
+             const sg_opts* o, const double* wd, const double* beta, const double* q, const double* w, double* out) {
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  const size_t nc = s.geo.nic * s.geo.njc, nv = (size_t)s.prm.nvar();
+  // beta enters only the SA production, linearly: dR/dbeta.w = P_c/beta_c*w_c/V_c
+  // is evaluated as a finite combination of two residuals, which is exact for
+  // a linear dependence up to rounding.
+  std::vector<double> b0(nc, 1.0), b1(nc);
+  if (beta) b0.assign(beta, beta + nc);
+  for (size_t c = 0; c < nc; ++c) b1[c] = b0[c] + w[c];
+  std::vector<double> r0(nc * nv), r1(nc * nv);
+  Residual<double> res;
+  init_res(res, s, wd, b0.data());
+  res.evaluate(q, r0.data());
+  init_res(res, s, wd, b1.data());
+  res.evaluate(q, r1.data());
+  for (size_t m = 0; m < nc * nv; ++m) out[m] = r1[m] - r0[m];
+  return 0;
+}
+
+// Dense Jacobian (row-major, n x n) by n forward dual sweeps; tiny grids only.
+int og_jacobian(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs,
+                int nbc, const sg_opts* o, const double* wd, const double* beta, const double* q, double* J) {
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  Residual<Dual<1>> res;
+  init_res(res, s, wd, beta);
+  const size_t n = s.geo.nic * s.geo.njc * (size_t)s.prm.nvar();
+  std::vector<Dual<1>> qd(n), rd(n);
+  for (size_t c = 0; c < n; ++c) {
+    for (size_t m = 0; m < n; ++m) { qd[m].v = q[m]; qd[m].d[0] = (m == c) ? 1.0 : 0.0; }
+    res.evaluate(qd.data(), rd.data());
+    for (size_t m = 0; m < n; ++m) J[m * n + c] = rd[m].d[0];
+  }
+  return 0;
+}
+
+int og_dt(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs, int nbc,
+          const sg_opts* o, const double* q, double cfl, double* dt) {
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  local_dt(s.geo, s.prm, q, cfl, dt);
+  return 0;
+}
+
+int og_surface(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs,
+               int nbc, const sg_opts* o, const double* wd, const double* beta, const double* q, double* x,
+               double* cp, double* cf, double* forces) {
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  Residual<double> res;
+  init_res(res, s, wd, beta);
+  std::vector<double> r(s.geo.nic * s.geo.njc * (size_t)s.prm.nvar());
+  res.evaluate(q, r.data());
+  surface(s.geo, s.prm, res, q, (long)o->tail, x, cp, cf, forces);
+  return 0;
+}
+
+int og_wall_distance(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs,
+                     int nbc, const sg_opts* o, double* d) {
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  wall_distance(s.geo, s.bcs, d);
+  return 0;
+}
+
+// CPU baseline timing: median wall time (seconds) of `reps` residual
+// evaluations after `warm` warm-ups, with `threads` OpenMP threads.
+double og_time_residual(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs,
+                        const sg_bc* bcs, int nbc, const sg_opts* o, const double* q, int warm, int reps,
+                        int threads) {
+  if (threads > 0) omp_set_num_threads(threads);
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  Residual<double> res;
+  init_res(res, s, nullptr, nullptr);
+  std::vector<double> r(s.geo.nic * s.geo.njc * (size_t)s.prm.nvar());
+  for (int w = 0; w < warm; ++w) res.evaluate(q, r.data());
+  std::vector<double> t(reps > 0 ? reps : 1);
+  for (int k = 0; k < reps; ++k) {
+    auto t0 = std::chrono::steady_clock::now();
+    res.evaluate(q, r.data());
+    t[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  std::sort(t.begin(), t.end());
+  return t[t.size() / 2];
+}
+
+int og_max_threads() { return omp_get_max_threads(); }
+
+}  // extern "C"

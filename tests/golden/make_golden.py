@@ -1,0 +1,94 @@
+"""Writes the golden fixtures under tests/golden/ by running the compiled
+reference (oracle/_ref/ref_harness, built by `make -C oracle ref` from
+/root/reference) on small grids and seeded states.
+
+Each fixture .npz holds the case inputs (grid, freestream, ordered boundary
+list, options, state, J.v direction) and the reference's outputs:
+  R        calc_residual                       (src/model/eulerequation.cpp:202)
+  dt       calc_dt at cfl=3                    (:237)
+  grads    grad_{u,v,T}_{chi,eta}              (src/utils/mesh.cpp:34)
+  surface  write_surface text (x, Cp, Cf)      (src/utils/io.cpp:181)
+  jv_fd    central difference of the reference residual along v
+           (h = 1e-6 * |q|-scaled), a derivative check of the tangent.
+Run:  python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, os.path.dirname(HERE))
+
+from structured_amd import cases, grids  # noqa: E402
+import oracle_ffi as of  # noqa: E402
+
+
+def wake_case():
+    """Small C-grid-like topology exercising the wake mirror: bottom wake on
+    ghosted 1..8 (mirrors onto 17..24), wall on 9..16, freestream around."""
+    xv, yv, _ = grids.flat_plate(25, 11, 1e-2)
+    bcs = [dict(type="freestream", face="top", start=0, end=-1),
+           dict(type="freestream", face="left", start=0, end=-1),
+           dict(type="freestream", face="right", start=0, end=-1),
+           dict(type="wake", face="bottom", start=1, end=8),
+           dict(type="wall", face="bottom", start=9, end=16, u=0.0, v=0.0)]
+    fs = dict(rho_inf=1.0, u_inf=0.5, v_inf=0.01, p_inf=1 / 1.4, T_inf=1 / 1.4, mu_inf=1e-3, pr_inf=0.7,
+              aoa_deg=1.0, nut_inf=0.0)
+    return cases.Case("wake_25x11", xv, yv, fs, bcs, order=2, turbulence=0, tail=9)
+
+
+def bump_iso_case():
+    xv, yv = grids.bump_channel(33, 17, 1e-3, 0.05, 0.2)
+    bcs = [dict(type="isothermalwall", face="bottom", start=1, end=-2, u=0.0, v=0.0, T=0.75),
+           dict(type="slipwall", face="top", start=1, end=-2),
+           dict(type="freestream", face="left", start=0, end=-1),
+           dict(type="outflow", face="right", start=0, end=-1)]
+    fs = dict(rho_inf=1.0, u_inf=0.2, v_inf=0.0, p_inf=1 / 1.4, T_inf=1 / 1.4, mu_inf=2e-5, pr_inf=0.72,
+              aoa_deg=0.0, nut_inf=0.0)
+    return cases.Case("bump_iso_33x17", xv, yv, fs, bcs, order=2, turbulence=0, tail=1)
+
+
+def all_cases():
+    out = [cases.channel_example(), cases.laminar_plate(25, 17), wake_case(), bump_iso_case()]
+    fo = cases.laminar_plate(21, 13)
+    fo.order = 1
+    fo.name = "laminar_plate_21x13_o1"
+    out.append(fo)
+    inv = cases.laminar_plate(21, 13)
+    inv.fs = dict(inv.fs, mu_inf=0.0)
+    inv.name = "inviscid_plate_21x13"
+    out.append(inv)
+    return out
+
+
+def main():
+    if not of.have_ref():
+        sys.exit("reference harness missing: make -C oracle ref")
+    index = {}
+    for case in all_cases():
+        q = case.perturbed_state(seed=7)
+        rng = np.random.default_rng(11)
+        v = rng.standard_normal(q.shape) * np.abs(q).max(axis=(0, 1))
+        ref = of.RefRun(case)
+        R = ref.residual(q)
+        dt = ref.dt(q, 3.0)
+        surf = ref.surface(q) if case.fs["mu_inf"] > 1e-15 else np.zeros((0, 3))
+        grads = ref.grads(q) if case.fs["mu_inf"] > 1e-15 else [np.zeros(0)] * 6
+        h = 1e-6
+        jv_fd = (ref.residual(q + h * v) - ref.residual(q - h * v)) / (2 * h)
+        path = os.path.join(HERE, case.name + ".npz")
+        np.savez_compressed(path, xv=case.xv, yv=case.yv, q=q, v=v, R=R, dt=dt, surface=surf, jv_fd=jv_fd,
+                            gu_c=grads[0], gv_c=grads[1], gT_c=grads[2], gu_e=grads[3], gv_e=grads[4],
+                            gT_e=grads[5])
+        index[case.name] = dict(fs=case.fs, bcs=case.bcs, order=case.order, turbulence=case.turbulence,
+                                tail=case.tail, dpdx=case.dpdx, dpdy=case.dpdy, cfl=3.0, fd_h=h)
+        print(case.name, "max|R| %.3e" % np.abs(R).max())
+    with open(os.path.join(HERE, "index.json"), "w") as f:
+        json.dump(index, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

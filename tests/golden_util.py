@@ -1,0 +1,23 @@
+"""Loads the committed golden fixtures (tests/golden/*.npz + index.json)."""
+import json
+import os
+
+import numpy as np
+
+from structured_amd import cases
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def names():
+    with open(os.path.join(HERE, "index.json")) as f:
+        return sorted(json.load(f).keys())
+
+
+def load(name):
+    with open(os.path.join(HERE, "index.json")) as f:
+        meta = json.load(f)[name]
+    z = np.load(os.path.join(HERE, name + ".npz"))
+    case = cases.Case(name, z["xv"], z["yv"], meta["fs"], meta["bcs"], order=meta["order"],
+                      turbulence=meta["turbulence"], tail=meta["tail"], dpdx=meta["dpdx"], dpdy=meta["dpdy"])
+    return case, {k: z[k] for k in z.files}, meta

@@ -1,0 +1,166 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and the compiled
+reference harness (oracle/_ref/ref_harness).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from structured_amd import abi, cases, grids
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _lib = C.CDLL(ORACLE_SO)
+        _lib.og_time_residual.restype = C.c_double
+    return _lib
+
+
+def _common(case):
+    xv = np.ascontiguousarray(case.xv, dtype=np.float64)
+    yv = np.ascontiguousarray(case.yv, dtype=np.float64)
+    fs = abi.Freestream(**case.fs)
+    bcs = abi.make_bcs(case.bcs)
+    o = abi.Opts(order=case.order, turbulence=case.turbulence, tail=case.tail, reserved=0, dpdx=case.dpdx,
+                 dpdy=case.dpdy)
+    return xv, yv, fs, bcs, o
+
+
+def _arr(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def residual(case, q, wd=None, beta=None):
+    xv, yv, fs, bcs, o = _common(case)
+    q = _arr(q)
+    wd, beta = _arr(wd), _arr(beta)
+    r = np.zeros_like(q)
+    l2 = np.zeros(case.nvar)
+    lib().og_residual(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs), C.byref(o),
+                      abi.dptr(wd), abi.dptr(beta), abi.dptr(q), abi.dptr(r), abi.dptr(l2))
+    return r, l2
+
+
+def jv(case, q, v, wd=None, beta=None):
+    xv, yv, fs, bcs, o = _common(case)
+    q, v, wd, beta = _arr(q), _arr(v), _arr(wd), _arr(beta)
+    r = np.zeros_like(q)
+    out = np.zeros_like(q)
+    lib().og_jv(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs), C.byref(o),
+                abi.dptr(wd), abi.dptr(beta), abi.dptr(q), abi.dptr(v), abi.dptr(r), abi.dptr(out))
+    return r, out
+
+
+def dbeta(case, q, w, wd=None, beta=None):
+    xv, yv, fs, bcs, o = _common(case)
+    q, w, wd, beta = _arr(q), _arr(w), _arr(wd), _arr(beta)
+    out = np.zeros_like(q)
+    lib().og_dbeta(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs), C.byref(o),
+                   abi.dptr(wd), abi.dptr(beta), abi.dptr(q), abi.dptr(w), abi.dptr(out))
+    return out
+
+
+def jacobian(case, q, wd=None, beta=None):
+    xv, yv, fs, bcs, o = _common(case)
+    q, wd, beta = _arr(q), _arr(wd), _arr(beta)
+    n = q.size
+    J = np.zeros((n, n))
+    lib().og_jacobian(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs), C.byref(o),
+                      abi.dptr(wd), abi.dptr(beta), abi.dptr(q), abi.dptr(J))
+    return J
+
+
+def dt(case, q, cfl):
+    xv, yv, fs, bcs, o = _common(case)
+    q = _arr(q)
+    out = np.zeros_like(q)
+    lib().og_dt(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs), C.byref(o),
+                abi.dptr(q), C.c_double(cfl), abi.dptr(out))
+    return out
+
+
+def surface(case, q, wd=None, beta=None):
+    xv, yv, fs, bcs, o = _common(case)
+    q, wd, beta = _arr(q), _arr(wd), _arr(beta)
+    nb = case.ni - 2 * case.tail + 1
+    x, cp, cf, forces = np.zeros(nb), np.zeros(nb), np.zeros(nb), np.zeros(6)
+    lib().og_surface(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs), C.byref(o),
+                     abi.dptr(wd), abi.dptr(beta), abi.dptr(q), abi.dptr(x), abi.dptr(cp), abi.dptr(cf),
+                     abi.dptr(forces))
+    return x, cp, cf, forces
+
+
+def wall_distance(case):
+    xv, yv, fs, bcs, o = _common(case)
+    d = np.zeros((case.nic, case.njc))
+    lib().og_wall_distance(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs),
+                           C.byref(o), abi.dptr(d))
+    return d
+
+
+def time_residual(case, q, warm=3, reps=20, threads=0):
+    xv, yv, fs, bcs, o = _common(case)
+    q = _arr(q)
+    return lib().og_time_residual(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs),
+                                  C.byref(o), abi.dptr(q), warm, reps, threads)
+
+
+# --------------------------------------------------------------- reference --
+def have_ref():
+    return os.path.exists(REF_HARNESS)
+
+
+class RefRun:
+    """Runs the compiled reference on a case inside a scratch directory."""
+
+    def __init__(self, case):
+        self.case = case
+        self.dir = tempfile.mkdtemp(prefix="sgref_")
+        grids.write_p3d(os.path.join(self.dir, "grid.p3d"), case.xv, case.yv)
+        with open(os.path.join(self.dir, "case.inp"), "w") as f:
+            f.write(cases.to_toml(case, "grid.p3d", label="case"))
+
+    def _run(self, *args):
+        subprocess.run([REF_HARNESS, "case.inp", *args], cwd=self.dir, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.PIPE)
+
+    def _state(self, q):
+        p = os.path.join(self.dir, "state.bin")
+        np.ascontiguousarray(q, dtype=np.float64).tofile(p)
+        return p
+
+    def residual(self, q):
+        self._run("res", self._state(q), "rhs.bin")
+        return np.fromfile(os.path.join(self.dir, "rhs.bin")).reshape(q.shape)
+
+    def dt(self, q, cfl):
+        self._run("dt", self._state(q), "%.17e" % cfl, "dt.bin")
+        return np.fromfile(os.path.join(self.dir, "dt.bin")).reshape(q.shape)
+
+    def surface(self, q):
+        self._run("surf", self._state(q))
+        return np.loadtxt(os.path.join(self.dir, "case.surface"))
+
+    def grads(self, q):
+        self._run("grad", self._state(q), "grad.bin")
+        c = self.case
+        raw = np.fromfile(os.path.join(self.dir, "grad.bin"))
+        nchi = c.ni * c.njc * 2
+        neta = c.nic * c.nj * 2
+        out, off = [], 0
+        for n, shp in [(nchi, (c.ni, c.njc, 2))] * 3 + [(neta, (c.nic, c.nj, 2))] * 3:
+            out.append(raw[off:off + n].reshape(shp))
+            off += n
+        return out

@@ -1,0 +1,60 @@
+"""The CPU oracle against the reference's own outputs (golden fixtures made
+by compiling the reference's sources, tests/golden/make_golden.py), and
+against the live compiled reference where oracle/_ref exists."""
+import numpy as np
+import pytest
+
+import golden_util as gu
+import oracle_ffi as of
+
+NAMES = gu.names()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_residual_matches_reference(name):
+    case, z, _ = gu.load(name)
+    r, l2 = of.residual(case, z["q"])
+    scale = np.abs(z["R"]).max()
+    assert np.abs(r - z["R"]).max() <= 1e-13 * scale
+    # Solver::step norm, src/solver/solver.cpp:125-134 (unnormalised)
+    np.testing.assert_allclose(l2, np.sqrt((z["R"] ** 2).sum(axis=(0, 1))), rtol=1e-13)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_dt_matches_reference(name):
+    case, z, meta = gu.load(name)
+    np.testing.assert_allclose(of.dt(case, z["q"], meta["cfl"]), z["dt"], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("name", [n for n in NAMES if not n.startswith("inviscid")])
+def test_surface_matches_reference(name):
+    case, z, _ = gu.load(name)
+    x, cp, cf, _ = of.surface(case, z["q"])
+    s = z["surface"]
+    # the reference writes the .surface file with 6 significant digits
+    np.testing.assert_allclose(x, s[:, 0], rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(cp, s[:, 1], rtol=2e-5, atol=1e-5 * np.abs(s[:, 1]).max())
+    np.testing.assert_allclose(cf, s[:, 2], rtol=2e-5, atol=1e-5 * np.abs(s[:, 2]).max())
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_jv_dual_matches_reference_fd(name):
+    """Dual-number J.v of the oracle against a central difference of the
+    compiled reference's residual (O(h^2) truncation + roundoff/h)."""
+    case, z, _ = gu.load(name)
+    _, jv = of.jv(case, z["q"], z["v"])
+    scale = np.abs(z["jv_fd"]).max()
+    assert np.abs(jv - z["jv_fd"]).max() <= 1e-6 * scale
+
+
+@pytest.mark.skipif(not of.have_ref(), reason="compiled reference (oracle/_ref) not built")
+def test_live_reference_seeds():
+    """Fresh seeds against the live compiled reference."""
+    for name in ("laminar_plate_25x17", "wake_25x11", "bump_iso_33x17"):
+        case, _, _ = gu.load(name)
+        ref = of.RefRun(case)
+        for seed in (3, 4):
+            q = case.perturbed_state(seed=seed, amp=0.02)
+            r, _ = of.residual(case, q)
+            rr = ref.residual(q)
+            assert np.abs(r - rr).max() <= 1e-13 * np.abs(rr).max()
