@@ -120,9 +120,13 @@ int sg_wall(sg_ctx* ctx, const double* q_aos, double* x, double* cp, double* cf,
 int sg_residual_dev(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_host);
 int sg_jv_dev(sg_ctx* ctx, const double* q_dev, const double* v_dev, double* jv_dev);
 int sg_jtv_dev(sg_ctx* ctx, const double* q_dev, const double* psi_dev, double* jtpsi_dev, double* dbeta_dev);
-/* Enqueue the residual only (no norm readback, no synchronisation). */
-int sg_residual_dev_async(sg_ctx* ctx, const double* q_dev, double* r_dev);
+/* Enqueue the residual (and, if l2_dev is not NULL, its per-equation L2 norm
+ * into device memory) without synchronising. */
+int sg_residual_dev_async(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_dev);
 int sg_synchronize(sg_ctx* ctx);
+/* The context's HIP stream (a hipStream_t), so callers can order their own
+ * work and events with the library's. */
+void* sg_get_stream(sg_ctx* ctx);
 /* Layout conversion between the reference AoS and the library SoA. */
 int sg_aos_to_soa_dev(sg_ctx* ctx, const double* aos_dev, double* soa_dev, int nvar);
 int sg_soa_to_aos_dev(sg_ctx* ctx, const double* soa_dev, double* aos_dev, int nvar);

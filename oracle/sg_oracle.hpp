@@ -514,19 +514,26 @@ struct Residual {
   }
 
   // SA-noft2 source at one interior cell, multiplied by the cell volume by the
-  // caller.  Cell gradients are the mean of the four face gradients.
+  // caller.  Cell gradients are Green-Gauss over the cell's four faces with
+  // the face value the arithmetic mean of the two adjacent cells (ghost cells
+  // across boundary faces) and outward area-weighted normals.
+  void cell_grad(const Arr2<T>& q, size_t i, size_t j, T* out) const {
+    const size_t gi = i + 1, gj = j + 1;
+    const Arr3<double>& nc = g->normal_chi; const Arr3<double>& ne = g->normal_eta;
+    T c = q(gi, gj);
+    T fe = 0.5 * (c + q(gi + 1, gj)), fw = 0.5 * (c + q(gi - 1, gj));
+    T fn = 0.5 * (c + q(gi, gj + 1)), fs = 0.5 * (c + q(gi, gj - 1));
+    const double V = g->volume(i, j);
+    for (int k = 0; k < 2; ++k)
+      out[k] = (fe * nc(i + 1, j, k) - fw * nc(i, j, k) + fn * ne(i, j + 1, k) - fs * ne(i, j, k)) / V;
+  }
   T sa_source(size_t i, size_t j) const {
     const size_t gi = i + 1, gj = j + 1;
     T r = rho(gi, gj), n = nt(gi, gj);
     T ml = prm.mu_inf * pow(Tt(gi, gj) / prm.T_inf, 2.0 / 3.0);
     T nu = ml / r;
     T g_u[2], g_v[2], g_n[2], g_r[2];
-    for (int k = 0; k < 2; ++k) {
-      g_u[k] = 0.25 * (gu_c(i, j, k) + gu_c(i + 1, j, k) + gu_e(i, j, k) + gu_e(i, j + 1, k));
-      g_v[k] = 0.25 * (gv_c(i, j, k) + gv_c(i + 1, j, k) + gv_e(i, j, k) + gv_e(i, j + 1, k));
-      g_n[k] = 0.25 * (gn_c(i, j, k) + gn_c(i + 1, j, k) + gn_e(i, j, k) + gn_e(i, j + 1, k));
-      g_r[k] = 0.25 * (gr_c(i, j, k) + gr_c(i + 1, j, k) + gr_e(i, j, k) + gr_e(i, j + 1, k));
-    }
+    cell_grad(u, i, j, g_u); cell_grad(v, i, j, g_v); cell_grad(nt, i, j, g_n); cell_grad(rho, i, j, g_r);
     T omega = fabs(g_v[0] - g_u[1]);
     const double d = wall_dist[i * g->njc + j], b = beta[i * g->njc + j];
     const double k2d2 = sa.kappa * sa.kappa * d * d;
@@ -574,7 +581,7 @@ struct Residual {
       face_grad(u, gu_c, gu_e); face_grad(v, gv_c, gv_e); face_grad(Tt, gT_c, gT_e);
       face_avg(u, ub_c, ub_e); face_avg(v, vb_c, vb_e); face_avg(Tt, Tb_c, Tb_e);
       face_avg(mu, mub_c, mub_e); face_avg(kk, kb_c, kb_e);
-      if (prm.sa) { face_grad(nt, gn_c, gn_e); face_grad(rho, gr_c, gr_e); face_avg(eta, etab_c, etab_e); }
+      if (prm.sa) { face_grad(nt, gn_c, gn_e); face_avg(eta, etab_c, etab_e); }
     }
     const Arr3<double>& nc = g->normal_chi; const Arr3<double>& ne = g->normal_eta;
     // convective fluxes

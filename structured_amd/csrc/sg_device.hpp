@@ -1,0 +1,59 @@
+// Plain-data descriptors shared by the host context and the kernels.
+#pragma once
+#include <cstddef>
+
+namespace sg {
+
+// Device geometry, struct-of-arrays.  Cell arrays: a[j*ldi + i] (i < nic,
+// j < njc).  Chi-face arrays (faces i = 0..ni-1 between cells i-1 and i):
+// a[j*ldf + i].  Eta-face arrays (faces j = 0..nj-1): a[j*ldi + i].
+struct DevGeom {
+  int ni, nj, nic, njc;
+  int ldi, ldf;
+  size_t plane;              // njc * ldi, stride between state components
+  const double* ncx;         // normal_chi x, [njc][ldf]
+  const double* ncy;
+  const double* nex;         // normal_eta x, [nj][ldi]
+  const double* ney;
+  const double* vol;         // [njc][ldi]
+  const double* wd;          // wall distance (SA)
+  const double* beta;        // SA production multiplier
+};
+
+struct DevPhys {
+  double rho_inf, u_inf, v_inf, p_inf, T_inf, mu_inf, pr, nut_inf;
+  double R, cp;              // gas constant p/(rho T) and cp = gamma R/(gamma-1)
+  double eps_chi, eps_eta;   // limiter epsilons (10/nic)^3, (10/njc)^3
+  double dpdx, dpdy;
+  int order;                 // 1 or 2
+  int viscous;               // mu_inf > 1e-15
+};
+
+// One boundary-condition evaluation.  Operand refs: >= 0 is an interior
+// cell (cj*nic + ci), < 0 is another node -(ref+1).
+enum BcOp { OP_FREESTREAM = 0, OP_SLIP = 1, OP_WALL = 2, OP_ISO = 3, OP_OUTFLOW = 4, OP_COPY = 5 };
+struct BcNode {
+  int op, ra, rb, pad;
+  double c0, c1, c2;         // (u_bc, v_bc, T_bc) or (nx, ny) of the wall face
+};
+
+// Ghost ring: perimeter index of ghosted cell (gi, gj):
+//   bottom row gj=0          : gi                      (gi = 0..nic+1)
+//   top row    gj=njc+1      : (nic+2) + gi
+//   left col   gi=0,  gj=1..njc : 2(nic+2) + gj-1
+//   right col  gi=nic+1       : 2(nic+2) + njc + gj-1
+// ghost_map[perimeter] = final node id, or -1 if no boundary writes it.
+struct DevBc {
+  const BcNode* nodes;
+  const int* ghost_map;
+};
+
+constexpr double kGamma = 1.4;
+
+// SA-noft2 constants (Spalart & Allmaras 1994 as catalogued by NASA TMR),
+// SA-neg branch for nu~ < 0 (c_t3 = 1.2, c_n1 = 16).
+constexpr double kCb1 = 0.1355, kSigma = 2.0 / 3.0, kCb2 = 0.622, kKappa = 0.41;
+constexpr double kCw2 = 0.3, kCw3 = 2.0, kCv1 = 7.1, kCv2 = 0.7, kCv3 = 0.9, kPrt = 0.9;
+constexpr double kCt3 = 1.2, kCn1 = 16.0;
+
+}  // namespace sg

@@ -1,0 +1,176 @@
+// Kernel launchers of the residual / tangent path and the small helper
+// kernels (norm finalisation, local time step, layout conversion, wall
+// quantities).  Everything is FP64.
+#include <hip/hip_runtime.h>
+
+#include "sg_kernels.hpp"
+#include "sg_residual.hpp"
+
+namespace sg {
+
+namespace {
+inline dim3 tile_grid(const DevGeom& g) { return dim3((g.nic + OX - 1) / OX, (g.njc + OY - 1) / OY); }
+}  // namespace
+
+size_t residual_partial_count(const DevGeom& g) {
+  dim3 gr = tile_grid(g);
+  return (size_t)gr.x * gr.y;
+}
+
+hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
+                           double* partial, hipStream_t s) {
+  dim3 grid = tile_grid(g);
+  Field<double> fq{q};
+  OutField<double> fo{r};
+  if (sa) hipLaunchKernelGGL((residual_kernel<double, true>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial);
+  else hipLaunchKernelGGL((residual_kernel<double, false>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,
+                     double* r, double* jv, hipStream_t s) {
+  dim3 grid = tile_grid(g);
+  Field<Dual> fq{q, v};
+  OutField<Dual> fo{r, jv};
+  if (sa) hipLaunchKernelGGL((residual_kernel<Dual, true>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, nullptr);
+  else hipLaunchKernelGGL((residual_kernel<Dual, false>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, nullptr);
+  return hipGetLastError();
+}
+
+// Sum of the per-block partials in block order (deterministic), then sqrt.
+__global__ void norm_finalize_kernel(const double* partial, int nblocks, int nv, double* l2) {
+  __shared__ double red[256];
+  for (int k = 0; k < nv; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partial[(size_t)b * nv + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) l2[k] = sqrt(red[0]);
+    __syncthreads();
+  }
+}
+
+hipError_t launch_norm_finalize(const double* partial, int nblocks, int nv, double* l2, hipStream_t s) {
+  hipLaunchKernelGGL(norm_finalize_kernel, dim3(1), dim3(256), 0, s, partial, nblocks, nv, l2);
+  return hipGetLastError();
+}
+
+// EulerEquation::calc_dt, src/model/eulerequation.cpp:237-259.
+__global__ void dt_kernel(DevGeom g, double mu_inf, int nv, const double* q, const double* dsmin, double cfl, double* dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= g.nic) return;
+  const size_t c = (size_t)j * g.ldi + i;
+  double r = q[c], u = q[g.plane + c] / r, v = q[2 * g.plane + c] / r, rE = q[3 * g.plane + c];
+  double p = (rE - 0.5 * r * (u * u + v * v)) * (kGamma - 1.0);
+  double lam = sqrt(kGamma * p / r) + fabs(u) + fabs(v);
+  double len = dsmin[c];
+  double val = cfl / (lam / len + 2.0 * mu_inf / len / len);
+  for (int k = 0; k < nv; ++k) dt[k * g.plane + c] = val;
+}
+
+hipError_t launch_dt(const DevGeom& g, double mu_inf, int nv, const double* q, const double* dsmin, double cfl,
+                     double* dt, hipStream_t s) {
+  hipLaunchKernelGGL(dt_kernel, dim3((g.nic + 255) / 256, g.njc), dim3(256), 0, s, g, mu_inf, nv, q, dsmin, cfl, dt);
+  return hipGetLastError();
+}
+
+// AoS q[i][j][k] <-> SoA a[k][j][i]; one thread per cell.
+__global__ void aos_to_soa_kernel(int nic, int njc, int ldi, int nv, const double* aos, double* soa) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nic) return;
+  const size_t plane = (size_t)njc * ldi;
+  const double* src = aos + ((size_t)i * njc + j) * nv;
+  for (int k = 0; k < nv; ++k) soa[k * plane + (size_t)j * ldi + i] = src[k];
+}
+__global__ void soa_to_aos_kernel(int nic, int njc, int ldi, int nv, const double* soa, double* aos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nic) return;
+  const size_t plane = (size_t)njc * ldi;
+  double* dst = aos + ((size_t)i * njc + j) * nv;
+  for (int k = 0; k < nv; ++k) dst[k] = soa[k * plane + (size_t)j * ldi + i];
+}
+
+hipError_t launch_aos_to_soa(int nic, int njc, int ldi, int nv, const double* aos, double* soa, hipStream_t s) {
+  hipLaunchKernelGGL(aos_to_soa_kernel, dim3((nic + 127) / 128, njc), dim3(128), 0, s, nic, njc, ldi, nv, aos, soa);
+  return hipGetLastError();
+}
+hipError_t launch_soa_to_aos(int nic, int njc, int ldi, int nv, const double* soa, double* aos, hipStream_t s) {
+  hipLaunchKernelGGL(soa_to_aos_kernel, dim3((nic + 127) / 128, njc), dim3(128), 0, s, nic, njc, ldi, nv, soa, aos);
+  return hipGetLastError();
+}
+
+// Wall quantities, IOManager::write_surface (src/utils/io.cpp:181-255): one
+// block; each thread takes wall cells, recomputes the eta-face (j = 0)
+// Green-Gauss gradients of u, v (the reference reads them from the last
+// residual evaluation), Cp from the first two interior cells and Cf from
+// mu_inf (u_y - v_x); the four force sums are reduced in a fixed order.
+template <bool SA>
+__global__ void wall_kernel(DevGeom g, DevPhys ph, DevBc bc, const double* q, int first, int nb, const double* xv0,
+                            const double* yv0, double qinf, double* cp_out, double* cf_out, double* fsum) {
+  __shared__ double red[4][256];
+  Field<double> fq{q};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int nic = g.nic;
+  for (int n = threadIdx.x; n < nb; n += blockDim.x) {
+    const int ci = first + n;
+    auto prim = [&](int a, int b) -> Prim<double> {
+      if (a >= 0 && a < nic && b >= 0 && b < g.njc) return prim_interior<double, SA>(fq, g, ph, a, b);
+      return eval_node<double, SA>(bc.ghost_map[ghost_index(a, b, nic, g.njc)], bc, fq, g, ph);
+    };
+    Prim<double> c0 = prim(ci, 0), c1 = prim(ci, 1);
+    double cp = (0.5 * (c0.p + c1.p) - ph.p_inf) / qinf;
+    // eta face j = 0 of column ci (src/utils/mesh.cpp:99-104,124-125)
+    Prim<double> s0 = prim(ci, -1), w0 = prim(ci - 1, 0), ws = prim(ci - 1, -1), e0 = prim(ci + 1, 0),
+                 es = prim(ci + 1, -1);
+    auto NX = [&](int a, int b) { return g.nex[(size_t)b * g.ldi + a]; };
+    auto NY = [&](int a, int b) { return g.ney[(size_t)b * g.ldi + a]; };
+    auto CX = [&](int a, int b) { return g.ncx[(size_t)b * g.ldf + a]; };
+    auto CY = [&](int a, int b) { return g.ncy[(size_t)b * g.ldf + a]; };
+    const double tx = 0.5 * (NX(ci, 0) + NX(ci, 1)), ty = 0.5 * (NY(ci, 0) + NY(ci, 1));
+    const double bx = NX(ci, 0), by = NY(ci, 0), lx = CX(ci, 0), ly = CY(ci, 0), rx = CX(ci + 1, 0), ry = CY(ci + 1, 0);
+    const double V = g.vol[ci];
+    auto grad = [&](double t, double b, double w, double wsv, double e, double esv, double* o) {
+      double l = 0.25 * (t + b + w + wsv), r = 0.25 * (t + b + e + esv);
+      o[0] = (tx * t - bx * b + rx * r - lx * l) / V;
+      o[1] = (ty * t - by * b + ry * r - ly * l) / V;
+    };
+    double gu[2], gv[2];
+    grad(c0.u, s0.u, w0.u, ws.u, e0.u, es.u, gu);
+    grad(c0.v, s0.v, w0.v, ws.v, e0.v, es.v, gv);
+    double tau = ph.mu_inf * (gu[1] - gv[0]) / qinf;
+    cp_out[n] = cp;
+    cf_out[n] = tau;
+    const double dx = xv0[ci + 1] - xv0[ci], dy = yv0[ci + 1] - yv0[ci];
+    double sfdiv = 2.0 / 3.0 * (gu[0] + gv[1]);
+    double sxx = ph.mu_inf * (2.0 * gu[0] - sfdiv) / qinf;
+    double syy = ph.mu_inf * (2.0 * gv[1] - sfdiv) / qinf;
+    acc[0] += -cp * dx;
+    acc[1] += cp * dy;
+    acc[2] += -tau * dy + syy * dx;
+    acc[3] += tau * dx - sxx * dy;
+  }
+  for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) fsum[threadIdx.x] = red[threadIdx.x][0];
+}
+
+hipError_t launch_wall(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, int first,
+                       int nb, const double* xv0, const double* yv0, double qinf, double* cp, double* cf, double* fsum,
+                       hipStream_t s) {
+  if (sa) hipLaunchKernelGGL((wall_kernel<true>), dim3(1), dim3(256), 0, s, g, ph, bc, q, first, nb, xv0, yv0, qinf, cp,
+                             cf, fsum);
+  else hipLaunchKernelGGL((wall_kernel<false>), dim3(1), dim3(256), 0, s, g, ph, bc, q, first, nb, xv0, yv0, qinf, cp,
+                          cf, fsum);
+  return hipGetLastError();
+}
+
+}  // namespace sg

@@ -1,0 +1,23 @@
+// Internal C++ interface between the host context and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sg_device.hpp"
+
+namespace sg {
+
+size_t residual_partial_count(const DevGeom& g);
+hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
+                           double* partial, hipStream_t s);
+hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,
+                     double* r, double* jv, hipStream_t s);
+hipError_t launch_norm_finalize(const double* partial, int nblocks, int nv, double* l2, hipStream_t s);
+hipError_t launch_dt(const DevGeom& g, double mu_inf, int nv, const double* q, const double* dsmin, double cfl,
+                     double* dt, hipStream_t s);
+hipError_t launch_aos_to_soa(int nic, int njc, int ldi, int nv, const double* aos, double* soa, hipStream_t s);
+hipError_t launch_soa_to_aos(int nic, int njc, int ldi, int nv, const double* soa, double* aos, hipStream_t s);
+hipError_t launch_wall(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, int first,
+                       int nb, const double* xv0, const double* yv0, double qinf, double* cp, double* cf, double* fsum,
+                       hipStream_t s);
+
+}  // namespace sg

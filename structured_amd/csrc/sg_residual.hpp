@@ -1,0 +1,524 @@
+// Fused finite-volume residual on LDS tiles (one pass per tile):
+// primitive recovery + ghost boundary cells -> power-law mu/k (+ SA eddy
+// viscosity) -> MUSCL/van-Albada-type reconstruction -> Roe face flux ->
+// Green-Gauss face gradients and viscous flux -> SA source -> assembly /V,
+// with the per-block partial sums of R^2 for the L2 norm.
+//
+// Follows EulerEquation::calc_residual (src/model/eulerequation.cpp:202-232)
+// and the kernels it calls: FluidModel::primvars (src/model/fluid.cpp:50-67),
+// BoundaryContainer::apply (src/model/bc.cpp:430-433), ReconstructionSecondOrder
+// (src/model/reconstruction.cpp:55-154), ConvectiveFluxRoe (src/model/flux.cpp:51-146),
+// Mesh::calc_gradient / calc_face (src/utils/mesh.cpp:8-131) and
+// DiffusiveFluxGreenGauss (src/model/flux.cpp:12-48).
+//
+// Tiling.  A 256-thread block is a 32 x 8 grid of face owners: thread
+// (tx, ty) computes the chi face i0+tx on row j0+ty and the eta face j0+ty on
+// column i0+tx, so every thread evaluates exactly two faces.  The block then
+// assembles the 31 x 7 cells whose four faces it owns; tiles advance by
+// (31, 7), so one column and one row of faces per tile are computed twice.
+// The cell data of the tile plus its halo (two cells along each face normal
+// for MUSCL, one cell around for the viscous stencil) is staged once in LDS
+// as primitives and transport properties.
+#pragma once
+#include "sg_device.hpp"
+#include "sg_scalar.hpp"
+
+namespace sg {
+
+constexpr int TX = 32, TY = 8, NT = TX * TY;
+constexpr int RW = TX + 3, RH = TY + 3, NPOS = RW * RH;  // LDS region, origin (i0-2, j0-2)
+constexpr int OX = TX - 1, OY = TY - 1;                    // output cells per tile
+
+template <class S>
+struct Prim {
+  S r, u, v, p, T, n;
+};
+
+template <class S, bool SA>
+__device__ __forceinline__ Prim<S> prim_interior(const Field<S>& q, const DevGeom& g, const DevPhys& ph, int ci, int cj) {
+  const size_t idx = (size_t)cj * g.ldi + ci;
+  Prim<S> P;
+  P.r = q[idx];
+  P.u = q[g.plane + idx] / P.r;
+  P.v = q[2 * g.plane + idx] / P.r;
+  P.p = (q[3 * g.plane + idx] - 0.5 * P.r * (P.u * P.u + P.v * P.v)) * (kGamma - 1.0);
+  P.T = P.p / P.r / ph.R;
+  if (SA) P.n = q[4 * g.plane + idx] / P.r;
+  else P.n = S(0.0);
+  return P;
+}
+
+// src/model/bc.cpp: freestream :25-60, slipwall :77-128, wall :149-204,
+// outflow :283-311, periodic/wake copies :223-263,328-365, isothermal :387-427.
+template <class S>
+__device__ __forceinline__ Prim<S> apply_op(const BcNode& nd, const Prim<S>& A, const Prim<S>& B, const DevPhys& ph) {
+  Prim<S> o;
+  switch (nd.op) {
+    case OP_FREESTREAM:
+      o.r = S(ph.rho_inf); o.u = S(ph.u_inf); o.v = S(ph.v_inf); o.p = S(ph.p_inf);
+      o.T = o.p / o.r / ph.R; o.n = S(ph.nut_inf);
+      break;
+    case OP_SLIP: {
+      const double nx = nd.c0, ny = nd.c1, ds = nx * nx + ny * ny;
+      o.p = 1.5 * A.p - 0.5 * B.p;
+      o.r = 1.5 * A.r - 0.5 * B.r;
+      S un = A.u * nx + A.v * ny;
+      o.u = A.u - 2.0 * un * nx / ds;
+      o.v = A.v - 2.0 * un * ny / ds;
+      o.T = o.p / o.r / ph.R;
+      o.n = 1.5 * A.n - 0.5 * B.n;
+    } break;
+    case OP_WALL:
+      o.T = 1.5 * A.T - 0.5 * B.T;
+      o.r = 1.5 * A.r - 0.5 * B.r;
+      o.u = 2.0 * nd.c0 - (1.5 * A.u - 0.5 * B.u);
+      o.v = 2.0 * nd.c1 - (1.5 * A.v - 0.5 * B.v);
+      o.p = o.r * ph.R * o.T;
+      o.n = -(1.5 * A.n - 0.5 * B.n);
+      break;
+    case OP_ISO:
+      o.p = 1.5 * A.p - 0.5 * B.p;
+      o.u = 2.0 * nd.c0 - (1.5 * A.u - 0.5 * B.u);
+      o.v = 2.0 * nd.c1 - (1.5 * A.v - 0.5 * B.v);
+      o.T = S(nd.c2);
+      o.r = o.p / o.T / ph.R;
+      o.n = -(1.5 * A.n - 0.5 * B.n);
+      break;
+    case OP_OUTFLOW:
+      o.r = A.r; o.u = A.u; o.v = A.v; o.p = S(ph.p_inf);
+      o.T = o.p / o.r / ph.R; o.n = A.n;
+      break;
+    default:  // OP_COPY
+      o = A;
+      break;
+  }
+  return o;
+}
+
+template <class S, bool SA>
+__device__ __forceinline__ Prim<S> ref_leaf(int ref, const Field<S>& q, const DevGeom& g, const DevPhys& ph) {
+  return prim_interior<S, SA>(q, g, ph, ref % g.nic, ref / g.nic);
+}
+
+// Node whose operands are interior cells.
+template <class S, bool SA>
+__device__ Prim<S> eval_leaf_node(const BcNode& nd, const Field<S>& q, const DevGeom& g, const DevPhys& ph) {
+  Prim<S> A, B;
+  if (nd.op != OP_FREESTREAM) A = ref_leaf<S, SA>(nd.ra, q, g, ph);
+  if (nd.op == OP_SLIP || nd.op == OP_WALL || nd.op == OP_ISO) B = ref_leaf<S, SA>(nd.rb, q, g, ph);
+  return apply_op<S>(nd, A, B, ph);
+}
+
+// Node whose operands are interior cells or leaf nodes (depth <= 2 is
+// guaranteed by the host-side boundary compiler).
+template <class S, bool SA>
+__device__ Prim<S> eval_node(int id, const DevBc& bc, const Field<S>& q, const DevGeom& g, const DevPhys& ph) {
+  const BcNode nd = bc.nodes[id];
+  Prim<S> A, B;
+  if (nd.op != OP_FREESTREAM) {
+    A = nd.ra >= 0 ? ref_leaf<S, SA>(nd.ra, q, g, ph) : eval_leaf_node<S, SA>(bc.nodes[-(nd.ra + 1)], q, g, ph);
+  }
+  if (nd.op == OP_SLIP || nd.op == OP_WALL || nd.op == OP_ISO) {
+    B = nd.rb >= 0 ? ref_leaf<S, SA>(nd.rb, q, g, ph) : eval_leaf_node<S, SA>(bc.nodes[-(nd.rb + 1)], q, g, ph);
+  }
+  return apply_op<S>(nd, A, B, ph);
+}
+
+__device__ __forceinline__ int ghost_index(int ci, int cj, int nic, int njc) {
+  const int gi = ci + 1, gj = cj + 1;
+  if (gj == 0) return gi;
+  if (gj == njc + 1) return (nic + 2) + gi;
+  if (gi == 0) return 2 * (nic + 2) + gj - 1;
+  return 2 * (nic + 2) + njc + gj - 1;
+}
+
+// LDS slots of the staged cell data.
+enum { L_R = 0, L_U, L_V, L_P, L_T, L_MU, L_K, L_N, L_ETA, L_ML, L_MAX };
+
+// MUSCL left/right states, src/model/reconstruction.cpp:142-219.
+// Left state of a face from cell c with neighbours cm (behind) and cp (ahead):
+//   dm = c - cm, dp = cp - c, s = 0.25 (3 dp dm + eps) / (2 (dp - dm)^2 + 3 dp dm + eps)
+//   left  = c + s (2/3 dm + 4/3 dp);   right = c - s (4/3 dm + 2/3 dp)
+template <class S>
+__device__ __forceinline__ S muscl_left(S cm, S c, S cp, double eps) {
+  const double thm = 2.0 / 3.0, thp = 4.0 / 3.0;
+  S dm = c - cm, dp = cp - c;
+  S a1 = 3.0 * dp * dm;
+  S a2 = 2.0 * (dp - dm) * (dp - dm) + a1;
+  S s = 0.25 * (a1 + eps) / (a2 + eps);
+  return c + s * (thm * dm + thp * dp);
+}
+template <class S>
+__device__ __forceinline__ S muscl_right(S cm, S c, S cp, double eps) {
+  const double thm = 2.0 / 3.0, thp = 4.0 / 3.0;
+  S dm = c - cm, dp = cp - c;
+  S a1 = 3.0 * dp * dm;
+  S a2 = 2.0 * (dp - dm) * (dp - dm) + a1;
+  S s = 0.25 * (a1 + eps) / (a2 + eps);
+  return c - s * (thp * dm + thm * dp);
+}
+
+// Roe flux without entropy fix, src/model/flux.cpp:51-146; with SA the
+// rho*nu~ transport is first-order upwind on the Roe mass flux.
+template <class S, bool SA>
+__device__ __forceinline__ void roe_flux(double nx, double ny, S rl, S ul, S vl, S pl, S rr, S ur, S vr, S pr, S ntl,
+                                         S ntr, S* f) {
+  const double gm1 = kGamma - 1.0, ogm1 = 1.0 / gm1;
+  S rli = 1.0 / rl, rul = rl * ul, rvl = rl * vl, uvl = 0.5 * (ul * ul + vl * vl);
+  S el = pl * ogm1 + rl * uvl, hl = (el + pl) * rli;
+  S rri = 1.0 / rr, rur = rr * ur, rvr = rr * vr, uvr = 0.5 * (ur * ur + vr * vr);
+  S er = pr * ogm1 + rr * uvr, hr = (er + pr) * rri;
+  S rat = ssqrt(rr * rli), rati = 1.0 / (rat + 1.0);
+  S rav = rat * rl, uav = (rat * ur + ul) * rati, vav = (rat * vr + vl) * rati, hav = (rat * hr + hl) * rati;
+  S uv = 0.5 * (uav * uav + vav * vav);
+  S cav = ssqrt(gm1 * (hav - uv));
+  S d1 = rr - rl, d2 = ur - ul, d3 = vr - vl, d4 = pr - pl;
+  const double dr = sqrt(nx * nx + ny * ny), r1 = nx / dr, r2 = ny / dr;
+  S uu = r1 * uav + r2 * vav, c2 = cav * cav, c2i = 1.0 / c2;
+  S auu = sabs(uu), aupc = sabs(uu + cav), aumc = sabs(uu - cav);
+  S uul = r1 * ul + r2 * vl, uur = r1 * ur + r2 * vr;
+  S rcav = rav * cav, aquu = uur - uul, c2ih = 0.5 * c2i, ruuav = auu * rav;
+  S b1 = auu * (d1 - c2i * d4);
+  S b2 = c2ih * aupc * (d4 + rcav * aquu);
+  S b3 = c2ih * aumc * (d4 - rcav * aquu);
+  S b4 = b1 + b2 + b3, b5 = cav * (b2 - b3);
+  S b6 = ruuav * (d2 - r1 * aquu), b7 = ruuav * (d3 - r2 * aquu);
+  S a2 = uav * b4 + r1 * b5 + b6, a3 = vav * b4 + r2 * b5 + b7;
+  S a4 = hav * b4 + uu * b5 + uav * b6 + vav * b7 - c2 * b1 * ogm1;
+  const double aj = 0.5 * dr;
+  S plar = pl + pr, epl = el + pl, epr = er + pr;
+  f[0] = aj * (rl * uul + rr * uur - b4);
+  f[1] = aj * (rul * uul + rur * uur + r1 * plar - a2);
+  f[2] = aj * (rvl * uul + rvr * uur + r2 * plar - a3);
+  f[3] = aj * (epl * uul + epr * uur - a4);
+  if (SA) f[4] = 0.5 * (f[0] * (ntl + ntr) - sabs(f[0]) * (ntr - ntl));
+}
+
+template <class S>
+struct FaceGeo {
+  double tx, ty, bx, by, lx, ly, rx, ry, vol;  // top/bottom/left/right normals, auxiliary volume
+};
+
+// Green-Gauss face gradient (src/utils/mesh.cpp:34-131) of one variable from
+// the face's left/right cells and its top/bottom 4-cell averages.
+template <class S>
+__device__ __forceinline__ void gg_grad(const FaceGeo<S>& G, S qt, S qb, S ql, S qr, S* out) {
+  out[0] = (G.tx * qt - G.bx * qb + G.rx * qr - G.lx * ql) / G.vol;
+  out[1] = (G.ty * qt - G.by * qb + G.ry * qr - G.ly * ql) / G.vol;
+}
+
+// DiffusiveFluxGreenGauss::evaluate, src/model/flux.cpp:12-48 (+ SA nu~ diffusion).
+template <class S, bool SA>
+__device__ __forceinline__ void visc_flux(double nx, double ny, const S* gu, const S* gv, const S* gT, S ub, S vb, S m,
+                                          S k, const S* gn, S et, S* f) {
+  S txy = m * (gu[1] + gv[0]);
+  S txx = m * (2.0 * gu[0] - 2.0 / 3.0 * (gu[0] + gv[1]));
+  S tyy = m * (2.0 * gv[1] - 2.0 / 3.0 * (gu[0] + gv[1]));
+  S qx = -k * gT[0], qy = -k * gT[1];
+  f[1] = txx * nx + txy * ny;
+  f[2] = txy * nx + tyy * ny;
+  f[3] = (ub * txx + vb * txy - qx) * nx + (ub * txy + vb * tyy - qy) * ny;
+  if (SA) f[4] = (1.0 / kSigma) * et * (gn[0] * nx + gn[1] * ny);
+}
+
+template <class S, bool SA>
+__device__ __forceinline__ void transport(const Prim<S>& P, const DevPhys& ph, S* slot) {
+  S ml = ph.mu_inf * spow23(P.T / ph.T_inf);
+  if (!SA) {
+    slot[L_MU] = ml;
+    slot[L_K] = ml * ph.cp / ph.pr;
+    return;
+  }
+  S rn = P.r * P.n;
+  S chi = rn / ml, chi3 = chi * chi * chi;
+  S mut(0.0), fn(1.0);
+  if (val(P.n) >= 0.0) mut = rn * (chi3 / (chi3 + kCv1 * kCv1 * kCv1));
+  else fn = (kCn1 + chi3) / (kCn1 - chi3);
+  slot[L_MU] = ml + mut;
+  slot[L_K] = ph.cp * (ml / ph.pr + mut / kPrt);
+  slot[L_ETA] = ml + rn * fn;
+  slot[L_ML] = ml;
+}
+
+// SA-noft2 source (production * beta - destruction + cb2 and density-gradient
+// diffusion terms, NASA TMR compressible form) at one cell; the cell
+// gradients are Green-Gauss over its four faces with arithmetic face means.
+template <class S>
+__device__ __forceinline__ S sa_source(S r, S n, S ml, const S* gu, const S* gv, const S* gn, const S* gr, double d,
+                                       double b) {
+  const double cw1 = kCb1 / (kKappa * kKappa) + (1.0 + kCb2) / kSigma;
+  S nu = ml / r;
+  S omega = sabs(gv[0] - gu[1]);
+  const double k2d2 = kKappa * kKappa * d * d;
+  S P, D;
+  if (val(n) >= 0.0) {
+    S chi = n / nu, chi3 = chi * chi * chi;
+    S fv1 = chi3 / (chi3 + kCv1 * kCv1 * kCv1);
+    S fv2 = 1.0 - chi / (1.0 + chi * fv1);
+    S sbar = n * fv2 / k2d2;
+    S stil = (val(sbar) >= -kCv2 * val(omega))
+                 ? omega + sbar
+                 : omega + omega * (kCv2 * kCv2 * omega + kCv3 * sbar) / ((kCv3 - 2.0 * kCv2) * omega - sbar);
+    S rr = (val(stil) > 0.0) ? smin(n / (stil * k2d2), 10.0) : S(10.0);
+    S r2 = rr * rr, r6 = r2 * r2 * r2;
+    S gg = rr + kCw2 * (r6 - rr);
+    S g2 = gg * gg, g6 = g2 * g2 * g2;
+    const double cw3_6 = kCw3 * kCw3 * kCw3 * kCw3 * kCw3 * kCw3;
+    S fw = gg * spow((1.0 + cw3_6) / (g6 + cw3_6), 1.0 / 6.0);
+    P = b * kCb1 * stil * r * n;
+    D = cw1 * fw * r * (n / d) * (n / d);
+  } else {
+    P = b * kCb1 * (1.0 - kCt3) * omega * r * n;
+    D = -cw1 * r * (n / d) * (n / d);
+  }
+  S cb2t = (kCb2 / kSigma) * r * (gn[0] * gn[0] + gn[1] * gn[1]);
+  S cross = -(1.0 / kSigma) * (nu + n) * (gr[0] * gn[0] + gr[1] * gn[1]);
+  return P - D + cb2t + cross;
+}
+
+template <class S, bool SA>
+__global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, DevBc bc, Field<S> q, OutField<S> out,
+                                                       double* partial) {
+  constexpr int NL = SA ? L_MAX : (L_K + 1);
+  constexpr int NV = SA ? 5 : 4;
+  constexpr int NLDS = NL * NPOS > 2 * NV * NT ? NL * NPOS : 2 * NV * NT;
+  __shared__ double lds_raw[NLDS * (sizeof(S) / sizeof(double))];
+  S* lds = reinterpret_cast<S*>(lds_raw);
+  __shared__ double red[NT / 64][NV];
+
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int i0 = blockIdx.x * OX, j0 = blockIdx.y * OY;
+  const int nic = g.nic, njc = g.njc;
+  const double nan = __builtin_nan("");
+
+  // ---- stage primitives (+ transport properties) of tile + halo in LDS
+  for (int p = tid; p < NPOS; p += NT) {
+    const int lx = p % RW, ly = p / RW;
+    const int ci = i0 - 2 + lx, cj = j0 - 2 + ly;
+    S slot[L_MAX];
+    const bool interior = ci >= 0 && ci < nic && cj >= 0 && cj < njc;
+    const bool ring = !interior && ci >= -1 && ci <= nic && cj >= -1 && cj <= njc;
+    int node = -1;
+    if (ring) node = bc.ghost_map[ghost_index(ci, cj, nic, njc)];
+    if (interior || node >= 0) {
+      Prim<S> P = interior ? prim_interior<S, SA>(q, g, ph, ci, cj) : eval_node<S, SA>(node, bc, q, g, ph);
+      slot[L_R] = P.r; slot[L_U] = P.u; slot[L_V] = P.v; slot[L_P] = P.p; slot[L_T] = P.T;
+      if (SA) slot[L_N] = P.n;
+      const bool box = lx >= 1 && lx < RW - 1 && ly >= 1 && ly < RH - 1;
+      if (ph.viscous && box) transport<S, SA>(P, ph, slot);
+      else { slot[L_MU] = S(nan); slot[L_K] = S(nan); if (SA) { slot[L_ETA] = S(nan); slot[L_ML] = S(nan); } }
+    } else {
+#pragma unroll
+      for (int k = 0; k < L_MAX; ++k) slot[k] = S(nan);
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) lds[k * NPOS + p] = slot[k];
+  }
+  __syncthreads();
+
+  auto at = [&](int k, int lx, int ly) -> S { return lds[k * NPOS + ly * RW + lx]; };
+  auto ncx = [&](int fi, int cj) { return g.ncx[(size_t)cj * g.ldf + fi]; };
+  auto ncy = [&](int fi, int cj) { return g.ncy[(size_t)cj * g.ldf + fi]; };
+  auto nex = [&](int ci, int fj) { return g.nex[(size_t)fj * g.ldi + ci]; };
+  auto ney = [&](int ci, int fj) { return g.ney[(size_t)fj * g.ldi + ci]; };
+  auto vol = [&](int ci, int cj) { return g.vol[(size_t)cj * g.ldi + ci]; };
+
+  S hc[NV], he[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) { hc[k] = S(0.0); he[k] = S(0.0); }
+
+  // ---- chi face fi = i0+tx on row cj = j0+ty (between cells fi-1 and fi)
+  {
+    const int fi = i0 + tx, cj = j0 + ty;
+    if (fi <= nic && cj < njc) {
+      const int xl = tx + 1, xr = tx + 2, y = ty + 2;  // LDS coords of the left/right cells
+      S st[2][5];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        S cl = at(L_R + k, xl, y), cr = at(L_R + k, xr, y);
+        if (ph.order == 2) {
+          st[0][k] = (fi >= 1) ? muscl_left(at(L_R + k, xl - 1, y), cl, cr, ph.eps_chi) : cl;
+          st[1][k] = (fi <= nic - 1) ? muscl_right(cl, cr, at(L_R + k, xr + 1, y), ph.eps_chi) : cr;
+        } else {
+          st[0][k] = cl; st[1][k] = cr;
+        }
+      }
+      S ntl(0.0), ntr(0.0);
+      if (SA) { ntl = at(L_N, xl, y); ntr = at(L_N, xr, y); }
+      const double nx = ncx(fi, cj), ny = ncy(fi, cj);
+      roe_flux<S, SA>(nx, ny, st[0][0], st[0][1], st[0][2], st[0][3], st[1][0], st[1][1], st[1][2], st[1][3], ntl, ntr,
+                      hc);
+      if (ph.viscous) {
+        FaceGeo<S> G;
+        if (fi == 0) {
+          G.tx = nex(0, cj + 1); G.ty = ney(0, cj + 1); G.bx = nex(0, cj); G.by = ney(0, cj);
+          G.rx = 0.5 * (ncx(0, cj) + ncx(1, cj)); G.ry = 0.5 * (ncy(0, cj) + ncy(1, cj));
+          G.lx = ncx(0, cj); G.ly = ncy(0, cj); G.vol = vol(0, cj);
+        } else if (fi == nic) {
+          G.tx = nex(nic - 1, cj + 1); G.ty = ney(nic - 1, cj + 1); G.bx = nex(nic - 1, cj); G.by = ney(nic - 1, cj);
+          G.rx = ncx(nic, cj); G.ry = ncy(nic, cj);
+          G.lx = 0.5 * (ncx(nic, cj) + ncx(nic - 1, cj)); G.ly = 0.5 * (ncy(nic, cj) + ncy(nic - 1, cj));
+          G.vol = vol(nic - 1, cj);
+        } else {
+          G.tx = 0.5 * (nex(fi, cj + 1) + nex(fi - 1, cj + 1)); G.ty = 0.5 * (ney(fi, cj + 1) + ney(fi - 1, cj + 1));
+          G.bx = 0.5 * (nex(fi, cj) + nex(fi - 1, cj)); G.by = 0.5 * (ney(fi, cj) + ney(fi - 1, cj));
+          G.rx = 0.5 * (ncx(fi, cj) + ncx(fi + 1, cj)); G.ry = 0.5 * (ncy(fi, cj) + ncy(fi + 1, cj));
+          G.lx = 0.5 * (ncx(fi, cj) + ncx(fi - 1, cj)); G.ly = 0.5 * (ncy(fi, cj) + ncy(fi - 1, cj));
+          G.vol = 0.5 * (vol(fi, cj) + vol(fi - 1, cj));
+        }
+        // left/right cells and their north/south neighbours
+        auto grad_avg = [&](int k, S* gr, S* avg) {
+          S a = at(k, xl, y), b = at(k, xr, y);
+          S qt = 0.25 * (a + b + at(k, xl, y + 1) + at(k, xr, y + 1));
+          S qb = 0.25 * (a + b + at(k, xl, y - 1) + at(k, xr, y - 1));
+          if (gr) gg_grad(G, qt, qb, a, b, gr);
+          if (avg) *avg = 0.25 * (a + b + qt + qb);
+        };
+        S gu[2], gv[2], gT[2], gn[2] = {S(0.0), S(0.0)}, ub, vb, mb, kb, eb(0.0);
+        grad_avg(L_U, gu, &ub);
+        grad_avg(L_V, gv, &vb);
+        grad_avg(L_T, gT, nullptr);
+        grad_avg(L_MU, nullptr, &mb);
+        grad_avg(L_K, nullptr, &kb);
+        if (SA) { grad_avg(L_N, gn, nullptr); grad_avg(L_ETA, nullptr, &eb); }
+        S fv[5];
+        visc_flux<S, SA>(nx, ny, gu, gv, gT, ub, vb, mb, kb, gn, eb, fv);
+#pragma unroll
+        for (int k = 1; k < NV; ++k) hc[k] = hc[k] - fv[k];
+      }
+    }
+  }
+
+  // ---- eta face fj = j0+ty on column ci = i0+tx (between cells fj-1 and fj)
+  {
+    const int fj = j0 + ty, ci = i0 + tx;
+    if (fj <= njc && ci < nic) {
+      const int x = tx + 2, ys = ty + 1, yn = ty + 2;
+      S st[2][5];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        S cs = at(L_R + k, x, ys), cn = at(L_R + k, x, yn);
+        if (ph.order == 2) {
+          st[0][k] = (fj >= 1) ? muscl_left(at(L_R + k, x, ys - 1), cs, cn, ph.eps_eta) : cs;
+          st[1][k] = (fj <= njc - 1) ? muscl_right(cs, cn, at(L_R + k, x, yn + 1), ph.eps_eta) : cn;
+        } else {
+          st[0][k] = cs; st[1][k] = cn;
+        }
+      }
+      S nts(0.0), ntn(0.0);
+      if (SA) { nts = at(L_N, x, ys); ntn = at(L_N, x, yn); }
+      const double nx = nex(ci, fj), ny = ney(ci, fj);
+      roe_flux<S, SA>(nx, ny, st[0][0], st[0][1], st[0][2], st[0][3], st[1][0], st[1][1], st[1][2], st[1][3], nts, ntn,
+                      he);
+      if (ph.viscous) {
+        FaceGeo<S> G;
+        if (fj == 0) {
+          G.tx = 0.5 * (nex(ci, 0) + nex(ci, 1)); G.ty = 0.5 * (ney(ci, 0) + ney(ci, 1));
+          G.bx = nex(ci, 0); G.by = ney(ci, 0);
+          G.lx = ncx(ci, 0); G.ly = ncy(ci, 0); G.rx = ncx(ci + 1, 0); G.ry = ncy(ci + 1, 0);
+          G.vol = vol(ci, 0);
+        } else if (fj == njc) {
+          G.tx = nex(ci, njc); G.ty = ney(ci, njc);
+          G.bx = 0.5 * (nex(ci, njc) + nex(ci, njc - 1)); G.by = 0.5 * (ney(ci, njc) + ney(ci, njc - 1));
+          G.lx = ncx(ci, njc - 1); G.ly = ncy(ci, njc - 1); G.rx = ncx(ci + 1, njc - 1); G.ry = ncy(ci + 1, njc - 1);
+          G.vol = vol(ci, njc - 1);
+        } else {
+          G.tx = 0.5 * (nex(ci, fj) + nex(ci, fj + 1)); G.ty = 0.5 * (ney(ci, fj) + ney(ci, fj + 1));
+          G.bx = 0.5 * (nex(ci, fj) + nex(ci, fj - 1)); G.by = 0.5 * (ney(ci, fj) + ney(ci, fj - 1));
+          G.lx = 0.5 * (ncx(ci, fj) + ncx(ci, fj - 1)); G.ly = 0.5 * (ncy(ci, fj) + ncy(ci, fj - 1));
+          G.rx = 0.5 * (ncx(ci + 1, fj) + ncx(ci + 1, fj - 1)); G.ry = 0.5 * (ncy(ci + 1, fj) + ncy(ci + 1, fj - 1));
+          G.vol = 0.5 * (vol(ci, fj) + vol(ci, fj - 1));
+        }
+        auto grad_avg = [&](int k, S* gr, S* avg) {
+          S t = at(k, x, yn), b = at(k, x, ys);
+          S l = 0.25 * (t + b + at(k, x - 1, yn) + at(k, x - 1, ys));
+          S r = 0.25 * (t + b + at(k, x + 1, yn) + at(k, x + 1, ys));
+          if (gr) gg_grad(G, t, b, l, r, gr);
+          if (avg) *avg = 0.25 * (l + r + t + b);
+        };
+        S gu[2], gv[2], gT[2], gn[2] = {S(0.0), S(0.0)}, ub, vb, mb, kb, eb(0.0);
+        grad_avg(L_U, gu, &ub);
+        grad_avg(L_V, gv, &vb);
+        grad_avg(L_T, gT, nullptr);
+        grad_avg(L_MU, nullptr, &mb);
+        grad_avg(L_K, nullptr, &kb);
+        if (SA) { grad_avg(L_N, gn, nullptr); grad_avg(L_ETA, nullptr, &eb); }
+        S fv[5];
+        visc_flux<S, SA>(nx, ny, gu, gv, gT, ub, vb, mb, kb, gn, eb, fv);
+#pragma unroll
+        for (int k = 1; k < NV; ++k) he[k] = he[k] - fv[k];
+      }
+    }
+  }
+
+  // ---- SA source of the owned output cell (needs the staged cells)
+  const int ci = i0 + tx, cj = j0 + ty;
+  const bool owns = tx < OX && ty < OY && ci < nic && cj < njc;
+  S src(0.0);
+  if (SA && owns && ph.viscous) {
+    const int x = tx + 2, y = ty + 2;
+    const double ex = ncx(ci + 1, cj), ey = ncy(ci + 1, cj), wx = ncx(ci, cj), wy = ncy(ci, cj);
+    const double nx_ = nex(ci, cj + 1), ny_ = ney(ci, cj + 1), sx = nex(ci, cj), sy = ney(ci, cj);
+    const double V = vol(ci, cj);
+    auto cgrad = [&](int k, S* o) {
+      S c = at(k, x, y);
+      S fe = 0.5 * (c + at(k, x + 1, y)), fw = 0.5 * (c + at(k, x - 1, y));
+      S fn = 0.5 * (c + at(k, x, y + 1)), fs = 0.5 * (c + at(k, x, y - 1));
+      o[0] = (fe * ex - fw * wx + fn * nx_ - fs * sx) / V;
+      o[1] = (fe * ey - fw * wy + fn * ny_ - fs * sy) / V;
+    };
+    S gu[2], gv[2], gn[2], gr[2];
+    cgrad(L_U, gu); cgrad(L_V, gv); cgrad(L_N, gn); cgrad(L_R, gr);
+    const size_t cidx = (size_t)cj * g.ldi + ci;
+    src = sa_source<S>(at(L_R, x, y), at(L_N, x, y), at(L_ML, x, y), gu, gv, gn, gr, g.wd[cidx], g.beta[cidx]);
+  }
+
+  // ---- exchange face fluxes through LDS (reusing the staging area)
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    lds[k * NT + tid] = hc[k];
+    lds[(NV + k) * NT + tid] = he[k];
+  }
+  __syncthreads();
+
+  double sq[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) sq[k] = 0.0;
+  if (owns) {
+    const double V = vol(ci, cj);
+    const size_t cidx = (size_t)cj * g.ldi + ci;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      S hw = lds[k * NT + tid], hE = lds[k * NT + tid + 1];
+      S hs = lds[(NV + k) * NT + tid], hn = lds[(NV + k) * NT + tid + TX];
+      S r = S(0.0) - (hn - hs);
+      r = r - (hE - hw);
+      if (k == 1) r = r + (-ph.dpdx * V);
+      if (k == 2) r = r + (-ph.dpdy * V);
+      if (SA && k == 4) r = r + V * src;
+      r = r / V;
+      out.put(k * g.plane + cidx, r);
+      sq[k] = val(r) * val(r);
+    }
+  }
+  // ---- block partial sums of R^2 (fixed order: wavefront butterfly, then waves)
+  if (partial) {
+    const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = sq[k];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      if (lane == 0) red[wv][k] = s;
+    }
+    __syncthreads();
+    if (tid < NV) {
+      double s = 0.0;
+      for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
+      partial[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * NV + tid] = s;
+    }
+  }
+}
+
+}  // namespace sg

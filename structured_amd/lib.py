@@ -1,0 +1,189 @@
+"""Python mirror of the C ABI (include/structured_gpu.h) over ctypes.
+
+``Equation`` mirrors the reference's EulerEquation surface used by the
+Newton loop (src/model/eulerequation.h:53-55): ``calc_residual``,
+``calc_dt`` plus the new tangent/adjoint products.  Arrays are numpy float64
+in the reference layout q[i][j][k].  The HIP library must be built
+(``python -m structured_amd.build``); without it every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libstructured_gpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "structured_gpu.h")
+
+ERRORS = {0: "SG_OK", -1: "SG_ERR_ARG", -2: "SG_ERR_BC", -3: "SG_ERR_NAN", -4: "SG_ERR_HIP", -5: "SG_ERR_NOCONV",
+          -6: "SG_ERR_IO", -7: "SG_ERR_STATE"}
+
+_lib = None
+
+
+class SgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("HIP library missing: build it with `python -m structured_amd.build`")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.sg_last_error.restype = C.c_char_p
+        _lib.sg_last_error.argtypes = [C.c_void_p]
+        _lib.sg_create.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                   C.c_void_p, C.POINTER(C.c_void_p)]
+        _lib.sg_destroy.argtypes = [C.c_void_p]
+        _lib.sg_destroy.restype = None
+        for name in ("sg_nvar", "sg_ldi"):
+            getattr(_lib, name).argtypes = [C.c_void_p]
+    return _lib
+
+
+def exported_symbols():
+    """Function names declared in include/structured_gpu.h."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(sg_[a-z0-9_]+)\s*\(", text)))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != tuple(shape):
+        a = a.reshape(shape)
+    return a
+
+
+class Equation:
+    """One grid + flow setup on the GPU (sg_create ... sg_destroy)."""
+
+    def __init__(self, case):
+        lib = load()
+        self.case = case
+        self.nic, self.njc, self.nvar = case.nic, case.njc, case.nvar
+        xv = _f64(case.xv)
+        yv = _f64(case.yv)
+        fs = abi.Freestream(**case.fs)
+        bcs = abi.make_bcs(case.bcs)
+        o = abi.Opts(order=case.order, turbulence=case.turbulence, tail=case.tail, reserved=0, dpdx=case.dpdx,
+                     dpdy=case.dpdy)
+        h = C.c_void_p()
+        rc = lib.sg_create(case.ni, case.nj, _p(xv), _p(yv), C.byref(fs), C.cast(bcs, C.c_void_p), len(case.bcs),
+                           C.byref(o), C.byref(h))
+        if rc != 0:
+            raise SgError(rc, lib.sg_last_error(None).decode())
+        self.h = h
+        self.ldi = lib.sg_ldi(h)
+        self.plane = self.njc * self.ldi
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().sg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SgError(rc, load().sg_last_error(self.h).decode())
+
+    @property
+    def shape(self):
+        return (self.nic, self.njc, self.nvar)
+
+    # ---- reference-shaped operations (host buffers, AoS)
+    def calc_residual(self, q):
+        q = _f64(q, self.shape)
+        r = np.empty_like(q)
+        l2 = np.zeros(self.nvar)
+        rc = load().sg_residual(self.h, _p(q), _p(r), _p(l2))
+        if rc not in (0, -3):
+            self._check(rc)
+        return r, l2
+
+    def jv(self, q, v):
+        q = _f64(q, self.shape)
+        v = _f64(v, self.shape)
+        out = np.empty_like(q)
+        self._check(load().sg_jv(self.h, _p(q), _p(v), _p(out)))
+        return out
+
+    def jtv(self, q, psi):
+        q = _f64(q, self.shape)
+        psi = _f64(psi, self.shape)
+        out = np.empty_like(q)
+        db = np.empty((self.nic, self.njc))
+        self._check(load().sg_jtv(self.h, _p(q), _p(psi), _p(out), _p(db)))
+        return out, db
+
+    def calc_dt(self, q, cfl):
+        q = _f64(q, self.shape)
+        out = np.empty_like(q)
+        self._check(load().sg_dt(self.h, _p(q), C.c_double(cfl), _p(out)))
+        return out
+
+    def wall(self, q):
+        q = _f64(q, self.shape)
+        nb = self.case.ni - 2 * self.case.tail + 1
+        x, cp, cf, f = np.empty(nb), np.empty(nb), np.empty(nb), np.empty(6)
+        self._check(load().sg_wall(self.h, _p(q), _p(x), _p(cp), _p(cf), _p(f)))
+        return x, cp, cf, f
+
+    def set_beta(self, beta):
+        beta = _f64(beta, (self.nic, self.njc))
+        self._check(load().sg_set_beta(self.h, _p(beta)))
+
+    def set_walldist(self, d):
+        d = _f64(d, (self.nic, self.njc))
+        self._check(load().sg_set_walldist(self.h, _p(d)))
+
+    def walldist(self):
+        d = np.empty((self.nic, self.njc))
+        self._check(load().sg_get_walldist(self.h, _p(d)))
+        return d
+
+    # ---- device-resident (SoA) operations on raw device pointers
+    def residual_dev_async(self, q_ptr: int, r_ptr: int, l2_ptr: int | None = None):
+        self._check(load().sg_residual_dev_async(self.h, C.c_void_p(q_ptr), C.c_void_p(r_ptr),
+                                                 None if l2_ptr is None else C.c_void_p(l2_ptr)))
+
+    def stream(self) -> int:
+        lib = load()
+        lib.sg_get_stream.restype = C.c_void_p
+        return int(lib.sg_get_stream(self.h) or 0)
+
+    def residual_dev(self, q_ptr: int, r_ptr: int):
+        lib = load()
+        l2 = np.zeros(self.nvar)
+        rc = lib.sg_residual_dev(self.h, C.c_void_p(q_ptr), C.c_void_p(r_ptr), _p(l2))
+        if rc not in (0, -3):
+            self._check(rc)
+        return l2
+
+    def jv_dev(self, q_ptr: int, v_ptr: int, out_ptr: int):
+        self._check(load().sg_jv_dev(self.h, C.c_void_p(q_ptr), C.c_void_p(v_ptr), C.c_void_p(out_ptr)))
+
+    def synchronize(self):
+        self._check(load().sg_synchronize(self.h))
+
+    def aos_to_soa_dev(self, aos_ptr: int, soa_ptr: int, nvar: int | None = None):
+        self._check(load().sg_aos_to_soa_dev(self.h, C.c_void_p(aos_ptr), C.c_void_p(soa_ptr), nvar or self.nvar))
+
+    def soa_to_aos_dev(self, soa_ptr: int, aos_ptr: int, nvar: int | None = None):
+        self._check(load().sg_soa_to_aos_dev(self.h, C.c_void_p(soa_ptr), C.c_void_p(aos_ptr), nvar or self.nvar))
