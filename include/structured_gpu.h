@@ -127,6 +127,67 @@ int sg_synchronize(sg_ctx* ctx);
 /* The context's HIP stream (a hipStream_t), so callers can order their own
  * work and events with the library's. */
 void* sg_get_stream(sg_ctx* ctx);
+/* ---- linear solver: the reference's interface (preallocate / set_lhs /
+ *      set_rhs / solve / solve_and_update, src/linearsolver/ls_eigen.h:25-31),
+ *      matrix-free.  set_lhs takes the linearisation state q and the local
+ *      time steps dt (as sg_dt returns them) and sets A = I/dt - J(q); it
+ *      also builds the preconditioner from the exact Jacobian's 13 stencil
+ *      blocks per cell (precond 0: none, 1: block Jacobi, 2: j-line
+ *      block-tridiagonal, 3: symmetric multicolour j-line Gauss-Seidel,
+ *      4: five-colour block ILU(0) of I/dt - J on the 13-point stencil). 
+ *      The Krylov method is restarted FGMRES (restart length `restart`,
+ *      at most `maxit` iterations, relative tolerance `rtol`). */
+int sg_ls_preallocate(sg_ctx* ctx, int restart, int precond, int maxit, double rtol);
+int sg_ls_set_lhs(sg_ctx* ctx, const double* q_aos, const double* dt_aos);
+int sg_ls_set_rhs(sg_ctx* ctx, const double* rhs_aos);
+int sg_ls_solve(sg_ctx* ctx, double* dq_aos, int* iters, double* relres);
+int sg_ls_solve_and_update(sg_ctx* ctx, double* q_aos, double under_relaxation);
+/* z = M^-1 v with the preconditioner built by the last set_lhs. */
+int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos);
+
+/* ---- nonlinear solve: Solver::solve / Solver::step, src/solver/solver.cpp:52-236.
+ *      Ramp semantics as the reference (after step `counter`, if
+ *      counter > ramp_iteration: cfl = min(cfl^exponent, cfl_max), UR =
+ *      min(UR^exponent, 10)); ser > 0 instead sets cfl = cfl0 *
+ *      (|R0|/|R|)^ser_exponent.  Stops when every equation's L2 norm is below
+ *      tolerance (SG_OK) or after iteration_max steps (SG_ERR_NOCONV). */
+typedef struct sg_newton_opts {
+  double cfl;
+  int cfl_ramp;
+  int cfl_ramp_iteration;
+  double cfl_ramp_exponent;
+  double cfl_max;
+  double under_relaxation;
+  int under_relaxation_ramp;
+  int under_relaxation_ramp_iteration;
+  double under_relaxation_ramp_exponent;
+  int iteration_max;
+  double tolerance;
+  int gmres_restart;
+  int gmres_maxit;
+  double gmres_rtol;
+  int precond;
+  int ser;
+  double ser_exponent;
+  int verbose;
+} sg_newton_opts;
+
+typedef struct sg_solve_report {
+  int iterations;
+  int converged;
+  int linear_iterations;
+  int reserved;
+  double l2_initial[5];
+  double l2_final[5];
+  double cfl_final;
+  double wall_s, residual_s, jacobian_s, linsolve_s;  /* Profiler buckets, src/utils/config.cpp:5-29 */
+} sg_solve_report;
+
+int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* opts, sg_solve_report* report);
+/* Per-iteration L2 norms (nvar per entry) and CFL of the last sg_solve;
+ * returns the number of entries recorded. */
+int sg_solve_history(sg_ctx* ctx, double* l2_hist, double* cfl_hist, int max_entries);
+
 /* Layout conversion between the reference AoS and the library SoA. */
 int sg_aos_to_soa_dev(sg_ctx* ctx, const double* aos_dev, double* soa_dev, int nvar);
 int sg_soa_to_aos_dev(sg_ctx* ctx, const double* soa_dev, double* aos_dev, int nvar);

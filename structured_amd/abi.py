@@ -41,3 +41,36 @@ def dptr(a):
     if a is None:
         return None
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [("cfl", C.c_double), ("cfl_ramp", C.c_int), ("cfl_ramp_iteration", C.c_int),
+                ("cfl_ramp_exponent", C.c_double), ("cfl_max", C.c_double), ("under_relaxation", C.c_double),
+                ("under_relaxation_ramp", C.c_int), ("under_relaxation_ramp_iteration", C.c_int),
+                ("under_relaxation_ramp_exponent", C.c_double), ("iteration_max", C.c_int), ("tolerance", C.c_double),
+                ("gmres_restart", C.c_int), ("gmres_maxit", C.c_int), ("gmres_rtol", C.c_double), ("precond", C.c_int),
+                ("ser", C.c_int), ("ser_exponent", C.c_double), ("verbose", C.c_int)]
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("linear_iterations", C.c_int),
+                ("reserved", C.c_int), ("l2_initial", C.c_double * 5), ("l2_final", C.c_double * 5),
+                ("cfl_final", C.c_double), ("wall_s", C.c_double), ("residual_s", C.c_double),
+                ("jacobian_s", C.c_double), ("linsolve_s", C.c_double)]
+
+
+# Defaults of the reference's [solver] table (src/utils/config.cpp:44-73) where
+# they exist; the Krylov settings and the tolerance test are this build's.
+NEWTON_DEFAULTS = dict(cfl=1.0, cfl_ramp=0, cfl_ramp_iteration=20, cfl_ramp_exponent=1.1, cfl_max=1e12,
+                       under_relaxation=1.0, under_relaxation_ramp=0, under_relaxation_ramp_iteration=20,
+                       under_relaxation_ramp_exponent=1.1, iteration_max=1, tolerance=1e-10, gmres_restart=30,
+                       gmres_maxit=60, gmres_rtol=1e-3, precond=2, ser=0, ser_exponent=1.0, verbose=0)
+
+
+def newton_opts(**kw):
+    d = dict(NEWTON_DEFAULTS)
+    unknown = set(kw) - set(d)
+    if unknown:
+        raise KeyError("unknown Newton options: %s" % sorted(unknown))
+    d.update(kw)
+    return NewtonOpts(**d)

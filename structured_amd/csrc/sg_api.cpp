@@ -376,6 +376,7 @@ int sg_create(int ni, int nj, const double* xv, const double* yv, const sg_frees
   ph.rho_inf = fs->rho_inf; ph.u_inf = fs->u_inf; ph.v_inf = fs->v_inf; ph.p_inf = fs->p_inf; ph.T_inf = fs->T_inf;
   ph.mu_inf = fs->mu_inf; ph.pr = fs->pr_inf; ph.nut_inf = fs->nut_inf;
   ph.R = fs->p_inf / fs->rho_inf / fs->T_inf;          // src/model/fluid.cpp:12
+  ph.iR = 1.0 / ph.R; ph.ipr = 1.0 / fs->pr_inf; ph.iTinf = 1.0 / fs->T_inf;
   ph.cp = kGamma * ph.R / (kGamma - 1.0);
   ph.eps_chi = std::pow(10.0 / G.nic, 3);               // src/model/reconstruction.cpp:62-63
   ph.eps_eta = std::pow(10.0 / G.njc, 3);
@@ -394,11 +395,11 @@ int sg_create(int ni, int nj, const double* xv, const double* yv, const sg_frees
   const size_t naos = ctx->ncell() * ctx->nvar, nsoa = ctx->nsoa();
   const size_t npart = residual_partial_count(g) * ctx->nvar;
   if (ctx->aos_a.alloc(naos) || ctx->aos_b.alloc(naos) || ctx->q.alloc(nsoa) || ctx->r.alloc(nsoa) ||
-      ctx->v.alloc(nsoa) || ctx->w.alloc(nsoa) || ctx->partial.alloc(npart) || ctx->l2.alloc(8) ||
+      ctx->v.alloc(nsoa) || ctx->w.alloc(nsoa) || ctx->partial.alloc(npart) || ctx->l2.alloc(8) || ctx->counter.alloc(1) ||
       ctx->wall_cp.alloc(G.ni) || ctx->wall_cf.alloc(G.ni) || ctx->wall_f.alloc(4))
     return fail(SG_ERR_HIP);
   // zero the SoA padding once so padded lanes never hold garbage
-  for (DevBuf* b : {&ctx->q, &ctx->r, &ctx->v, &ctx->w})
+  for (DevBuf* b : {&ctx->q, &ctx->r, &ctx->v, &ctx->w, &ctx->counter})
     if (hipMemset(b->p, 0, b->n * sizeof(double)) != hipSuccess) return fail(SG_ERR_HIP);
   *out = ctx;
   return SG_OK;
@@ -406,10 +407,9 @@ int sg_create(int ni, int nj, const double* xv, const double* yv, const sg_frees
 
 void sg_destroy(sg_ctx* ctx) {
   if (!ctx) return;
-  if (ctx->stream) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-  }
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  sg_solver_free(ctx);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
@@ -451,11 +451,9 @@ int sg_get_metrics(sg_ctx* ctx, double* nchi, double* neta, double* volume, doub
 
 int sg_residual_dev_async(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_dev) {
   if (!ctx || !q_dev || !r_dev) return SG_ERR_ARG;
-  SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, l2_dev ? ctx->partial.p : nullptr, ctx->stream),
+  SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, l2_dev ? ctx->partial.p : nullptr,
+                         reinterpret_cast<unsigned*>(ctx->counter.p), l2_dev, ctx->stream),
          "residual launch");
-  if (l2_dev)
-    SG_TRY(launch_norm_finalize(ctx->partial.p, (int)residual_partial_count(ctx->g), ctx->nvar, l2_dev, ctx->stream),
-           "norm launch");
   return SG_OK;
 }
 
@@ -463,9 +461,8 @@ void* sg_get_stream(sg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 int sg_residual_dev(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_host) {
   if (!ctx || !q_dev || !r_dev) return SG_ERR_ARG;
-  const int nb = (int)residual_partial_count(ctx->g);
-  SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, ctx->partial.p, ctx->stream), "residual launch");
-  SG_TRY(launch_norm_finalize(ctx->partial.p, nb, ctx->nvar, ctx->l2.p, ctx->stream), "norm launch");
+  SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, ctx->partial.p,
+                         reinterpret_cast<unsigned*>(ctx->counter.p), ctx->l2.p, ctx->stream), "residual launch");
   double l2[8];
   SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, ctx->nvar * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
   SG_TRY(hipStreamSynchronize(ctx->stream), "residual");
@@ -478,9 +475,8 @@ int sg_residual(sg_ctx* ctx, const double* q_aos, double* r_aos, double* l2) {
   int rc;
   if ((rc = upload_aos(ctx, q_aos, ctx->q.p))) return rc;
   double l2h[8];
-  const int nb = (int)residual_partial_count(ctx->g);
-  SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, ctx->q.p, ctx->r.p, ctx->partial.p, ctx->stream), "residual launch");
-  SG_TRY(launch_norm_finalize(ctx->partial.p, nb, ctx->nvar, ctx->l2.p, ctx->stream), "norm launch");
+  SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, ctx->q.p, ctx->r.p, ctx->partial.p,
+                         reinterpret_cast<unsigned*>(ctx->counter.p), ctx->l2.p, ctx->stream), "residual launch");
   SG_TRY(hipMemcpyAsync(l2h, ctx->l2.p, ctx->nvar * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
   if ((rc = download_aos(ctx, ctx->r.p, r_aos))) return rc;
   SG_TRY(hipStreamSynchronize(ctx->stream), "residual");

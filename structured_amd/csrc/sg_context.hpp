@@ -55,6 +55,8 @@ struct BcCompiled {
 
 }  // namespace sg
 
+struct SolverState;
+
 struct sg_ctx {
   sg::HostGeom geo;
   sg::BcCompiled bcc;
@@ -72,11 +74,13 @@ struct sg_ctx {
   sg::DevBuf nodes_buf, map_buf;   // BcNode / int storage (raw doubles)
   sg::DevBuf aos_a, aos_b;         // AoS staging (nvar * ncell)
   sg::DevBuf q, r, v, w;           // SoA work vectors (nvar * plane)
-  sg::DevBuf partial, l2;
+  sg::DevBuf partial, l2, counter;  // counter: one unsigned, kept at 0 between launches
   sg::DevBuf wall_cp, wall_cf, wall_f;
+  SolverState* solver = nullptr;   // Newton-Krylov work space (sg_solver.cpp)
   size_t ncell() const { return (size_t)geo.nic * geo.njc; }
   size_t nsoa() const { return (size_t)nvar * g.plane; }
 };
 
 int sg_fail(sg_ctx* ctx, int code, const std::string& msg);
+void sg_solver_free(sg_ctx* ctx);
 int sg_hip(sg_ctx* ctx, hipError_t e, const char* what);

@@ -23,6 +23,7 @@ struct DevGeom {
 struct DevPhys {
   double rho_inf, u_inf, v_inf, p_inf, T_inf, mu_inf, pr, nut_inf;
   double R, cp;              // gas constant p/(rho T) and cp = gamma R/(gamma-1)
+  double iR, ipr, iTinf;     // 1/R, 1/Pr, 1/T_inf
   double eps_chi, eps_eta;   // limiter epsilons (10/nic)^3, (10/njc)^3
   double dpdx, dpdy;
   int order;                 // 1 or 2

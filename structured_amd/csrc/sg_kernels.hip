@@ -18,13 +18,16 @@ size_t residual_partial_count(const DevGeom& g) {
 }
 
 hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
-                           double* partial, hipStream_t s) {
+                           double* partial, unsigned* counter, double* l2, hipStream_t s) {
   dim3 grid = tile_grid(g);
   Field<double> fq{q};
   OutField<double> fo{r};
+  (void)counter;
   if (sa) hipLaunchKernelGGL((residual_kernel<double, true>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial);
   else hipLaunchKernelGGL((residual_kernel<double, false>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial);
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !partial || !l2) return e;
+  return launch_norm_finalize(partial, (int)residual_partial_count(g), sa ? 5 : 4, l2, s);
 }
 
 hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,
@@ -37,21 +40,22 @@ hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool 
   return hipGetLastError();
 }
 
-// Sum of the per-block partials in block order (deterministic), then sqrt.
-__global__ void norm_finalize_kernel(const double* partial, int nblocks, int nv, double* l2) {
-  __shared__ double red[256];
+// Sum of the per-block partials (fixed assignment of partials to threads,
+// fixed butterfly and wave order, so the result is bitwise reproducible),
+// then sqrt.  One block, one pass over the partials.
+__global__ __launch_bounds__(256) void norm_finalize_kernel(const double* partial, int nblocks, int nv, double* l2) {
+  __shared__ double red[4][8];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += 256)
+    for (int k = 0; k < nv; ++k) acc[k] += partial[(size_t)b * nv + k];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int k = 0; k < nv; ++k) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partial[(size_t)b * nv + k];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) l2[k] = sqrt(red[0]);
-    __syncthreads();
+    double s = acc[k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) red[wv][k] = s;
   }
+  __syncthreads();
+  if ((int)threadIdx.x < nv) l2[threadIdx.x] = sqrt(red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 hipError_t launch_norm_finalize(const double* partial, int nblocks, int nv, double* l2, hipStream_t s) {

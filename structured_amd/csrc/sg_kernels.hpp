@@ -7,8 +7,11 @@
 namespace sg {
 
 size_t residual_partial_count(const DevGeom& g);
+// partial: per-block norm partials (or null); with l2 (device, nv entries)
+// the last block also finalises the norm.  counter must hold 0 on entry and
+// is left at 0.
 hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
-                           double* partial, hipStream_t s);
+                           double* partial, unsigned* counter, double* l2, hipStream_t s);
 hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,
                      double* r, double* jv, hipStream_t s);
 hipError_t launch_norm_finalize(const double* partial, int nblocks, int nv, double* l2, hipStream_t s);

@@ -1,0 +1,484 @@
+// Device linear algebra for the matrix-free Newton-Krylov solve:
+// deterministic multi-dot products (CGS2 Gram-Schmidt), multi-axpy, scaling,
+// colour seeding and extraction of Jacobian blocks from J.v products, and the
+// j-line block-tridiagonal preconditioner (factor + solve, one thread per
+// i-column).  All vectors are SoA with zero padding (length nvar*plane).
+#include <hip/hip_runtime.h>
+
+#include "sg_linalg.hpp"
+
+namespace sg {
+
+constexpr int DOT_BLOCK = 256;
+constexpr int DOT_GRID = 512;  // partial sums per dot product
+
+// out_partial[b*k + i] = sum over this block's slice of V_i . w
+__global__ __launch_bounds__(DOT_BLOCK) void multi_dot_kernel(const double* V, size_t ldv, int k, const double* w,
+                                                              size_t n, double* partial) {
+  __shared__ double red[DOT_BLOCK / 64][32];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int i0 = 0; i0 < k; i0 += 32) {
+    const int kk = min(32, k - i0);
+    double acc[32];
+    for (int i = 0; i < 32; ++i) acc[i] = 0.0;
+    for (size_t e = (size_t)blockIdx.x * DOT_BLOCK + threadIdx.x; e < n; e += (size_t)gridDim.x * DOT_BLOCK) {
+      const double we = w[e];
+      for (int i = 0; i < kk; ++i) acc[i] += V[(size_t)(i0 + i) * ldv + e] * we;
+    }
+    for (int i = 0; i < kk; ++i) {
+      double s = acc[i];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      if (lane == 0) red[wv][i] = s;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < kk) {
+      double s = 0.0;
+      for (int q = 0; q < DOT_BLOCK / 64; ++q) s += red[q][threadIdx.x];
+      partial[(size_t)blockIdx.x * k + i0 + threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// out[i] = sum_b partial[b*k + i]  (+ optional add into out when accumulate)
+__global__ void dot_finalize_kernel(const double* partial, int nb, int k, double* out, int accumulate) {
+  const int i = blockIdx.x;
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(size_t)b * k + i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0] + red[1] + red[2] + red[3];
+    out[i] = accumulate ? out[i] + t : t;
+  }
+}
+
+hipError_t multi_dot(const double* V, size_t ldv, int k, const double* w, size_t n, double* partial, double* out,
+                     bool accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(multi_dot_kernel, dim3(DOT_GRID), dim3(DOT_BLOCK), 0, s, V, ldv, k, w, n, partial);
+  hipLaunchKernelGGL(dot_finalize_kernel, dim3(k), dim3(256), 0, s, partial, DOT_GRID, k, out, accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+size_t multi_dot_partial_size(int kmax) { return (size_t)DOT_GRID * kmax; }
+
+// w -= sum_i c[i] V_i   (c in device memory)
+__global__ void multi_axpy_kernel(const double* V, size_t ldv, int k, const double* c, double* w, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    double s = w[e];
+    for (int i = 0; i < k; ++i) s -= c[i] * V[(size_t)i * ldv + e];
+    w[e] = s;
+  }
+}
+
+hipError_t multi_axpy(const double* V, size_t ldv, int k, const double* c, double* w, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(multi_axpy_kernel, dim3(1024), dim3(256), 0, s, V, ldv, k, c, w, n);
+  return hipGetLastError();
+}
+
+// y = sum_i c[i] X_i  (c on the host, k small)
+__global__ void lincomb_kernel(const double* X, size_t ldx, int k, const double* c, double* y, size_t n, int accumulate) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    double s = accumulate ? y[e] : 0.0;
+    for (int i = 0; i < k; ++i) s += c[i] * X[(size_t)i * ldx + e];
+    y[e] = s;
+  }
+}
+
+hipError_t lincomb(const double* X, size_t ldx, int k, const double* c_dev, double* y, size_t n, bool accumulate,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(lincomb_kernel, dim3(1024), dim3(256), 0, s, X, ldx, k, c_dev, y, n, accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+// y = a*x + b*y
+__global__ void axpby_kernel(double a, const double* x, double b, double* y, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    y[e] = a * x[e] + b * y[e];
+}
+hipError_t axpby(double a, const double* x, double b, double* y, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(axpby_kernel, dim3(1024), dim3(256), 0, s, a, x, b, y, n);
+  return hipGetLastError();
+}
+
+// y = x * scale[k*plane...] style: w = z * idt - jz  (operator I/dt - J)
+__global__ void shift_minus_kernel(const double* z, const double* idt, const double* jz, double* w, size_t plane, int nv) {
+  const size_t n = plane * nv;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    w[e] = z[e] * idt[e % plane] - jz[e];
+}
+hipError_t shift_minus(const double* z, const double* idt, const double* jz, double* w, size_t plane, int nv, hipStream_t s) {
+  hipLaunchKernelGGL(shift_minus_kernel, dim3(1024), dim3(256), 0, s, z, idt, jz, w, plane, nv);
+  return hipGetLastError();
+}
+
+// v = e_m on the cells of colour (a, b) = (i mod 5, j mod 5), 0 elsewhere.
+__global__ void seed_kernel(int nic, int njc, int ldi, int nv, int a, int b, int m, double* v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nic) return;
+  const size_t plane = (size_t)njc * ldi, c = (size_t)j * ldi + i;
+  const bool on = (i % 5 == a) && (j % 5 == b);
+  for (int k = 0; k < nv; ++k) v[k * plane + c] = (on && k == m) ? 1.0 : 0.0;
+}
+hipError_t seed_colour(int nic, int njc, int ldi, int nv, int a, int b, int m, double* v, hipStream_t s) {
+  hipLaunchKernelGGL(seed_kernel, dim3((nic + 127) / 128, njc), dim3(128), 0, s, nic, njc, ldi, nv, a, b, m, v);
+  return hipGetLastError();
+}
+
+// Stencil slots of the exact Jacobian of the residual: the 3x3 box plus
+// (0,+-2) and (+-2,0) (MUSCL reach).  Slot s holds J[r, r + (di,dj)].
+__constant__ int kSlotDi[NSLOT] = {0, 0, 0, -1, 1, -1, 1, -1, 1, 0, 0, -2, 2};
+__constant__ int kSlotDj[NSLOT] = {0, -1, 1, 0, 0, -1, -1, 1, 1, -2, 2, 0, 0};
+
+__device__ __forceinline__ int slot_of(int di, int dj) {
+  for (int s = 0; s < NSLOT; ++s)
+    if (kSlotDi[s] == di && kSlotDj[s] == dj) return s;
+  return -1;
+}
+
+// From J.v of colour (a,b), variable m: row cell r holds column m of the
+// block J[r, c] for the unique colour-(a,b) cell c in r's 5x5 window.
+// Blocks are struct-of-arrays: blocks[((slot*nv + row)*nv + col)*plane + cell].
+__global__ void extract_kernel(int nic, int njc, int ldi, int nv, int a, int b, int m, const double* jv, double* blocks) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nic) return;
+  int di = ((a - i) % 5 + 5) % 5;
+  if (di > 2) di -= 5;
+  int dj = ((b - j) % 5 + 5) % 5;
+  if (dj > 2) dj -= 5;
+  const int slot = slot_of(di, dj);
+  if (slot < 0) return;
+  const size_t plane = (size_t)njc * ldi, c = (size_t)j * ldi + i;
+  for (int k = 0; k < nv; ++k) blocks[((size_t)(slot * nv + k) * nv + m) * plane + c] = jv[k * plane + c];
+}
+hipError_t extract_blocks(int nic, int njc, int ldi, int nv, int a, int b, int m, const double* jv, double* blocks,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(extract_kernel, dim3((nic + 127) / 128, njc), dim3(128), 0, s, nic, njc, ldi, nv, a, b, m, jv, blocks);
+  return hipGetLastError();
+}
+
+// ---- small dense helpers (row-major nv x nv, nv <= 5) ----
+template <int NV>
+__device__ void inv_small(const double* A, double* Ainv) {
+  double M[NV][2 * NV];
+  for (int r = 0; r < NV; ++r)
+    for (int c = 0; c < NV; ++c) { M[r][c] = A[r * NV + c]; M[r][NV + c] = (r == c) ? 1.0 : 0.0; }
+  for (int p = 0; p < NV; ++p) {
+    int piv = p;
+    double best = fabs(M[p][p]);
+    for (int r = p + 1; r < NV; ++r)
+      if (fabs(M[r][p]) > best) { best = fabs(M[r][p]); piv = r; }
+    if (piv != p)
+      for (int c = 0; c < 2 * NV; ++c) { double t = M[p][c]; M[p][c] = M[piv][c]; M[piv][c] = t; }
+    const double ip = 1.0 / M[p][p];
+    for (int c = 0; c < 2 * NV; ++c) M[p][c] *= ip;
+    for (int r = 0; r < NV; ++r) {
+      if (r == p) continue;
+      const double f = M[r][p];
+      if (f != 0.0)
+        for (int c = 0; c < 2 * NV; ++c) M[r][c] -= f * M[p][c];
+    }
+  }
+  for (int r = 0; r < NV; ++r)
+    for (int c = 0; c < NV; ++c) Ainv[r * NV + c] = M[r][NV + c];
+}
+
+// Block-tridiagonal factor of each i-column of A = I/dt - J restricted to
+// the (0,-1), (0,0), (0,+1) blocks (jacobi = 0) or its block diagonal only
+// (jacobi = 1).  fac (SoA, 3 x NV*NV per cell): Dinv_j, C_j = Dinv_j U_j, L_j.
+template <int NV>
+__global__ void line_factor_kernel(int nic, int njc, int ldi, const double* idt, const double* blocks, double* fac,
+                                   int jacobi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nic) return;
+  const size_t plane = (size_t)njc * ldi;
+  constexpr int B = NV * NV;
+  double D[B], Dinv[B], Cprev[B], L[B];
+  for (int j = 0; j < njc; ++j) {
+    const size_t c = (size_t)j * ldi + i;
+    for (int e = 0; e < B; ++e) D[e] = -blocks[(size_t)(0 * B + e) * plane + c];
+    for (int k = 0; k < NV; ++k) D[k * NV + k] += idt[c];
+    if (!jacobi && j > 0) {
+      for (int e = 0; e < B; ++e) L[e] = -blocks[(size_t)(1 * B + e) * plane + c];
+      for (int r = 0; r < NV; ++r)
+        for (int cc = 0; cc < NV; ++cc) {
+          double s = 0.0;
+          for (int t = 0; t < NV; ++t) s += L[r * NV + t] * Cprev[t * NV + cc];
+          D[r * NV + cc] -= s;
+        }
+    } else {
+      for (int e = 0; e < B; ++e) L[e] = 0.0;
+    }
+    inv_small<NV>(D, Dinv);
+    for (int e = 0; e < B; ++e) {
+      fac[(size_t)(0 * B + e) * plane + c] = Dinv[e];
+      fac[(size_t)(2 * B + e) * plane + c] = L[e];
+    }
+    for (int r = 0; r < NV; ++r)
+      for (int cc = 0; cc < NV; ++cc) {
+        double s = 0.0;
+        if (!jacobi && j < njc - 1)
+          for (int t = 0; t < NV; ++t) s += Dinv[r * NV + t] * (-blocks[(size_t)(2 * B + t * NV + cc) * plane + c]);
+        Cprev[r * NV + cc] = s;
+        fac[(size_t)(1 * B + r * NV + cc) * plane + c] = s;
+      }
+  }
+}
+
+// Solve one factored line: x_line = T^-1 rhs_line (rhs in `t`, SoA).
+template <int NV>
+__device__ void line_substitute(int i, int njc, int ldi, size_t plane, const double* fac, const double* t, double* x) {
+  constexpr int B = NV * NV;
+  double y[NV];
+  for (int k = 0; k < NV; ++k) y[k] = 0.0;
+  for (int j = 0; j < njc; ++j) {
+    const size_t c = (size_t)j * ldi + i;
+    double r[NV];
+    for (int k = 0; k < NV; ++k) {
+      double s = t[k * plane + c];
+      for (int m = 0; m < NV; ++m) s -= fac[(size_t)(2 * B + k * NV + m) * plane + c] * y[m];
+      r[k] = s;
+    }
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int m = 0; m < NV; ++m) s += fac[(size_t)(0 * B + k * NV + m) * plane + c] * r[m];
+      y[k] = s;
+      x[k * plane + c] = s;
+    }
+  }
+  for (int j = njc - 2; j >= 0; --j) {
+    const size_t c = (size_t)j * ldi + i;
+    double cur[NV];
+    for (int k = 0; k < NV; ++k) {
+      double s = x[k * plane + c];
+      for (int m = 0; m < NV; ++m) s -= fac[(size_t)(1 * B + k * NV + m) * plane + c] * y[m];
+      cur[k] = s;
+    }
+    for (int k = 0; k < NV; ++k) { x[k * plane + c] = cur[k]; y[k] = cur[k]; }
+  }
+}
+
+template <int NV>
+__global__ void line_solve_kernel(int nic, int njc, int ldi, const double* fac, const double* b, double* x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nic) return;
+  line_substitute<NV>(i, njc, ldi, (size_t)njc * ldi, fac, b, x);
+}
+
+// One colour of a line Gauss-Seidel sweep: for the i-lines with i % 3 ==
+// colour, t = b - sum over the stencil blocks outside the line's tridiagonal
+// band of (-J) x, then x_line = T^-1 t.  Lines of one colour never couple
+// (the stencil reaches +-2 in i), so they run in parallel.
+template <int NV>
+__global__ void line_gs_kernel(int nic, int njc, int ldi, int colour, const double* blocks, const double* fac,
+                               const double* b, double* t, double* x) {
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) * 3 + colour;
+  if (i >= nic) return;
+  constexpr int B = NV * NV;
+  const size_t plane = (size_t)njc * ldi;
+  for (int j = 0; j < njc; ++j) {
+    const size_t c = (size_t)j * ldi + i;
+    double acc[NV];
+    for (int k = 0; k < NV; ++k) acc[k] = b[k * plane + c];
+    for (int s = 3; s < NSLOT; ++s) {
+      const int ii = i + kSlotDi[s], jj = j + kSlotDj[s];
+      if (ii < 0 || ii >= nic || jj < 0 || jj >= njc) continue;
+      const size_t cn = (size_t)jj * ldi + ii;
+      double xn[NV];
+      for (int m = 0; m < NV; ++m) xn[m] = x[m * plane + cn];
+      for (int k = 0; k < NV; ++k) {
+        double sum = 0.0;
+        for (int m = 0; m < NV; ++m) sum += blocks[(size_t)(s * B + k * NV + m) * plane + c] * xn[m];
+        acc[k] += sum;  // A = I/dt - J: off-diagonal blocks are -J
+      }
+    }
+    for (int k = 0; k < NV; ++k) t[k * plane + c] = acc[k];
+  }
+  line_substitute<NV>(i, njc, ldi, plane, fac, t, x);
+}
+
+hipError_t line_factor(int nic, int njc, int ldi, int nv, const double* idt, const double* blocks, double* fac,
+                       bool jacobi, hipStream_t s) {
+  dim3 g((nic + 63) / 64), b(64);
+  if (nv == 5) hipLaunchKernelGGL(line_factor_kernel<5>, g, b, 0, s, nic, njc, ldi, idt, blocks, fac, jacobi ? 1 : 0);
+  else hipLaunchKernelGGL(line_factor_kernel<4>, g, b, 0, s, nic, njc, ldi, idt, blocks, fac, jacobi ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t line_solve(int nic, int njc, int ldi, int nv, const double* fac, const double* bvec, double* x, hipStream_t s) {
+  dim3 g((nic + 63) / 64), b(64);
+  if (nv == 5) hipLaunchKernelGGL(line_solve_kernel<5>, g, b, 0, s, nic, njc, ldi, fac, bvec, x);
+  else hipLaunchKernelGGL(line_solve_kernel<4>, g, b, 0, s, nic, njc, ldi, fac, bvec, x);
+  return hipGetLastError();
+}
+
+hipError_t line_gs(int nic, int njc, int ldi, int nv, int colour, const double* blocks, const double* fac,
+                   const double* bvec, double* t, double* x, hipStream_t s) {
+  const int nl = (nic - colour + 2) / 3;
+  if (nl <= 0) return hipSuccess;
+  dim3 g((nl + 63) / 64), b(64);
+  if (nv == 5) hipLaunchKernelGGL(line_gs_kernel<5>, g, b, 0, s, nic, njc, ldi, colour, blocks, fac, bvec, t, x);
+  else hipLaunchKernelGGL(line_gs_kernel<4>, g, b, 0, s, nic, njc, ldi, colour, blocks, fac, bvec, t, x);
+  return hipGetLastError();
+}
+
+// ---- multicolour block ILU(0) on the 13-point stencil ---------------------
+// Colour c(i,j) = (i + 3j) mod 5: no two cells of one colour are stencil
+// neighbours, so with the colours ordered 0..4 the factorisation and both
+// triangular solves run one colour per launch, fully parallel inside it.
+// ilu (SoA, same layout as blocks) holds, per row r: slot 0 the inverse of
+// the factored diagonal block, slots to lower-colour neighbours L_rn, slots
+// to higher-colour neighbours U_rj.
+__device__ __forceinline__ int colour5(int i, int j) { return ((i + 3 * j) % 5 + 5) % 5; }
+
+__device__ __forceinline__ int slot_index(int di, int dj) {
+  // inverse of kSlotDi/kSlotDj
+  for (int s = 0; s < NSLOT; ++s)
+    if (kSlotDi[s] == di && kSlotDj[s] == dj) return s;
+  return -1;
+}
+
+template <int NV>
+__global__ void ilu_init_kernel(int nic, int njc, int ldi, const double* idt, const double* blocks, double* ilu) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nic) return;
+  constexpr int B = NV * NV;
+  const size_t plane = (size_t)njc * ldi, c = (size_t)j * ldi + i;
+  for (int s = 0; s < NSLOT; ++s)
+    for (int e = 0; e < B; ++e) {
+      double a = -blocks[(size_t)(s * B + e) * plane + c];
+      if (s == 0 && (e / NV) == (e % NV)) a += idt[c];
+      ilu[(size_t)(s * B + e) * plane + c] = a;
+    }
+}
+
+template <int NV>
+__global__ void ilu_factor_kernel(int nic, int njc, int ldi, int colour, double* ilu) {
+  const int j = blockIdx.y;
+  const int i = ((colour - 3 * j) % 5 + 5) % 5 + 5 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= nic) return;
+  constexpr int B = NV * NV;
+  const size_t plane = (size_t)njc * ldi, c = (size_t)j * ldi + i;
+  auto at = [&](int s, size_t cell) -> double* { return ilu + (size_t)(s * B) * plane + cell; };  // stride plane
+  for (int cn = 0; cn < colour; ++cn) {
+    for (int s = 1; s < NSLOT; ++s) {
+      const int di = kSlotDi[s], dj = kSlotDj[s];
+      const int ni_ = i + di, nj_ = j + dj;
+      if (ni_ < 0 || ni_ >= nic || nj_ < 0 || nj_ >= njc || colour5(ni_, nj_) != cn) continue;
+      const size_t cnb = (size_t)nj_ * ldi + ni_;
+      // L_rn = A_rn * Dinv_n
+      double Lb[B];
+      const double* Arn = at(s, c);
+      const double* Dn = at(0, cnb);
+      for (int r = 0; r < NV; ++r)
+        for (int q = 0; q < NV; ++q) {
+          double sum = 0.0;
+          for (int t = 0; t < NV; ++t) sum += Arn[(r * NV + t) * plane] * Dn[(t * NV + q) * plane];
+          Lb[r * NV + q] = sum;
+        }
+      double* Lout = at(s, c);
+      for (int e = 0; e < B; ++e) Lout[e * plane] = Lb[e];
+      // A_rj -= L_rn U_nj for the higher-colour neighbours j of n inside r's stencil
+      for (int t = 1; t < NSLOT; ++t) {
+        const int ji = ni_ + kSlotDi[t], jj = nj_ + kSlotDj[t];
+        if (ji < 0 || ji >= nic || jj < 0 || jj >= njc || colour5(ji, jj) <= cn) continue;
+        const int u = slot_index(di + kSlotDi[t], dj + kSlotDj[t]);
+        if (u < 0) continue;
+        const double* Unj = at(t, cnb);
+        double* Arj = at(u, c);
+        for (int r = 0; r < NV; ++r)
+          for (int q = 0; q < NV; ++q) {
+            double sum = 0.0;
+            for (int k = 0; k < NV; ++k) sum += Lb[r * NV + k] * Unj[(k * NV + q) * plane];
+            Arj[(r * NV + q) * plane] -= sum;
+          }
+      }
+    }
+  }
+  double D[B], Dinv[B];
+  double* Dd = at(0, c);
+  for (int e = 0; e < B; ++e) D[e] = Dd[e * plane];
+  inv_small<NV>(D, Dinv);
+  for (int e = 0; e < B; ++e) Dd[e * plane] = Dinv[e];
+}
+
+// forward (lower = 1): z_r = b_r - sum_{lower n} L_rn z_n
+// backward (lower = 0): z_r = Dinv_r (z_r - sum_{upper j} U_rj z_j)
+template <int NV>
+__global__ void ilu_solve_kernel(int nic, int njc, int ldi, int colour, int lower, const double* ilu, const double* b,
+                                 double* z) {
+  const int j = blockIdx.y;
+  const int i = ((colour - 3 * j) % 5 + 5) % 5 + 5 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= nic) return;
+  constexpr int B = NV * NV;
+  const size_t plane = (size_t)njc * ldi, c = (size_t)j * ldi + i;
+  double acc[NV];
+  for (int k = 0; k < NV; ++k) acc[k] = lower ? b[k * plane + c] : z[k * plane + c];
+  for (int s = 1; s < NSLOT; ++s) {
+    const int ni_ = i + kSlotDi[s], nj_ = j + kSlotDj[s];
+    if (ni_ < 0 || ni_ >= nic || nj_ < 0 || nj_ >= njc) continue;
+    const int cn = colour5(ni_, nj_);
+    if (lower ? (cn >= colour) : (cn <= colour)) continue;
+    const size_t cnb = (size_t)nj_ * ldi + ni_;
+    double zn[NV];
+    for (int m = 0; m < NV; ++m) zn[m] = z[m * plane + cnb];
+    for (int k = 0; k < NV; ++k) {
+      double sum = 0.0;
+      for (int m = 0; m < NV; ++m) sum += ilu[(size_t)(s * B + k * NV + m) * plane + c] * zn[m];
+      acc[k] -= sum;
+    }
+  }
+  if (lower) {
+    for (int k = 0; k < NV; ++k) z[k * plane + c] = acc[k];
+  } else {
+    for (int k = 0; k < NV; ++k) {
+      double sum = 0.0;
+      for (int m = 0; m < NV; ++m) sum += ilu[(size_t)(k * NV + m) * plane + c] * acc[m];
+      z[k * plane + c] = sum;
+    }
+  }
+}
+
+hipError_t ilu_factor(int nic, int njc, int ldi, int nv, const double* idt, const double* blocks, double* ilu,
+                      hipStream_t s) {
+  dim3 gi((nic + 127) / 128, njc), bi(128);
+  if (nv == 5) hipLaunchKernelGGL(ilu_init_kernel<5>, gi, bi, 0, s, nic, njc, ldi, idt, blocks, ilu);
+  else hipLaunchKernelGGL(ilu_init_kernel<4>, gi, bi, 0, s, nic, njc, ldi, idt, blocks, ilu);
+  const int per_row = (nic + 4) / 5;
+  dim3 g((per_row + 63) / 64, njc), b(64);
+  for (int c = 0; c < 5; ++c) {
+    if (nv == 5) hipLaunchKernelGGL(ilu_factor_kernel<5>, g, b, 0, s, nic, njc, ldi, c, ilu);
+    else hipLaunchKernelGGL(ilu_factor_kernel<4>, g, b, 0, s, nic, njc, ldi, c, ilu);
+  }
+  return hipGetLastError();
+}
+
+hipError_t ilu_apply(int nic, int njc, int ldi, int nv, const double* ilu, const double* bvec, double* z, hipStream_t s) {
+  const int per_row = (nic + 4) / 5;
+  dim3 g((per_row + 63) / 64, njc), b(64);
+  for (int c = 0; c < 5; ++c) {
+    if (nv == 5) hipLaunchKernelGGL(ilu_solve_kernel<5>, g, b, 0, s, nic, njc, ldi, c, 1, ilu, bvec, z);
+    else hipLaunchKernelGGL(ilu_solve_kernel<4>, g, b, 0, s, nic, njc, ldi, c, 1, ilu, bvec, z);
+  }
+  for (int c = 4; c >= 0; --c) {
+    if (nv == 5) hipLaunchKernelGGL(ilu_solve_kernel<5>, g, b, 0, s, nic, njc, ldi, c, 0, ilu, bvec, z);
+    else hipLaunchKernelGGL(ilu_solve_kernel<4>, g, b, 0, s, nic, njc, ldi, c, 0, ilu, bvec, z);
+  }
+  return hipGetLastError();
+}
+
+// inverse local time step per cell: idt = 1/dt (dt from the dt kernel, SoA plane 0)
+__global__ void invert_kernel(const double* dt, double* idt, int nic, int njc, int ldi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nic) return;
+  const size_t c = (size_t)j * ldi + i;
+  idt[c] = 1.0 / dt[c];
+}
+hipError_t invert_dt(const double* dt, double* idt, int nic, int njc, int ldi, hipStream_t s) {
+  hipLaunchKernelGGL(invert_kernel, dim3((nic + 127) / 128, njc), dim3(128), 0, s, dt, idt, nic, njc, ldi);
+  return hipGetLastError();
+}
+
+}  // namespace sg

@@ -1,0 +1,39 @@
+// Device linear algebra used by the Newton-Krylov driver (sg_linalg.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace sg {
+
+constexpr int NSLOT = 13;  // Jacobian stencil blocks kept per cell
+
+size_t multi_dot_partial_size(int kmax);
+// out[i] = V_i . w for i < k (V_i = V + i*ldv); out is device memory.
+hipError_t multi_dot(const double* V, size_t ldv, int k, const double* w, size_t n, double* partial, double* out,
+                     bool accumulate, hipStream_t s);
+// w -= sum_i c[i] V_i, c device memory
+hipError_t multi_axpy(const double* V, size_t ldv, int k, const double* c, double* w, size_t n, hipStream_t s);
+// y (+)= sum_i c[i] X_i, c device memory
+hipError_t lincomb(const double* X, size_t ldx, int k, const double* c_dev, double* y, size_t n, bool accumulate,
+                   hipStream_t s);
+hipError_t axpby(double a, const double* x, double b, double* y, size_t n, hipStream_t s);
+// w = z * idt (per cell) - jz
+hipError_t shift_minus(const double* z, const double* idt, const double* jz, double* w, size_t plane, int nv,
+                       hipStream_t s);
+hipError_t seed_colour(int nic, int njc, int ldi, int nv, int a, int b, int m, double* v, hipStream_t s);
+hipError_t extract_blocks(int nic, int njc, int ldi, int nv, int a, int b, int m, const double* jv, double* blocks,
+                          hipStream_t s);
+hipError_t line_factor(int nic, int njc, int ldi, int nv, const double* idt, const double* blocks, double* fac,
+                       bool jacobi, hipStream_t s);
+hipError_t line_solve(int nic, int njc, int ldi, int nv, const double* fac, const double* b, double* x, hipStream_t s);
+// symmetric multicolour line Gauss-Seidel colour step (see sg_linalg.hip)
+hipError_t line_gs(int nic, int njc, int ldi, int nv, int colour, const double* blocks, const double* fac,
+                   const double* bvec, double* t, double* x, hipStream_t s);
+// multicolour (5 colours) block ILU(0) of I/dt - J on the 13-point stencil
+hipError_t ilu_factor(int nic, int njc, int ldi, int nv, const double* idt, const double* blocks, double* ilu,
+                      hipStream_t s);
+hipError_t ilu_apply(int nic, int njc, int ldi, int nv, const double* ilu, const double* b, double* z, hipStream_t s);
+hipError_t invert_dt(const double* dt, double* idt, int nic, int njc, int ldi, hipStream_t s);
+
+}  // namespace sg

@@ -39,11 +39,12 @@ __device__ __forceinline__ Prim<S> prim_interior(const Field<S>& q, const DevGeo
   const size_t idx = (size_t)cj * g.ldi + ci;
   Prim<S> P;
   P.r = q[idx];
-  P.u = q[g.plane + idx] / P.r;
-  P.v = q[2 * g.plane + idx] / P.r;
+  const S ir = srcp(P.r);
+  P.u = q[g.plane + idx] * ir;
+  P.v = q[2 * g.plane + idx] * ir;
   P.p = (q[3 * g.plane + idx] - 0.5 * P.r * (P.u * P.u + P.v * P.v)) * (kGamma - 1.0);
-  P.T = P.p / P.r / ph.R;
-  if (SA) P.n = q[4 * g.plane + idx] / P.r;
+  P.T = P.p * ir * ph.iR;
+  if (SA) P.n = q[4 * g.plane + idx] * ir;
   else P.n = S(0.0);
   return P;
 }
@@ -56,16 +57,16 @@ __device__ __forceinline__ Prim<S> apply_op(const BcNode& nd, const Prim<S>& A, 
   switch (nd.op) {
     case OP_FREESTREAM:
       o.r = S(ph.rho_inf); o.u = S(ph.u_inf); o.v = S(ph.v_inf); o.p = S(ph.p_inf);
-      o.T = o.p / o.r / ph.R; o.n = S(ph.nut_inf);
+      o.T = sdiv(o.p, o.r) * ph.iR; o.n = S(ph.nut_inf);
       break;
     case OP_SLIP: {
-      const double nx = nd.c0, ny = nd.c1, ds = nx * nx + ny * ny;
+      const double nx = nd.c0, ny = nd.c1, ids = frcp(nx * nx + ny * ny);
       o.p = 1.5 * A.p - 0.5 * B.p;
       o.r = 1.5 * A.r - 0.5 * B.r;
       S un = A.u * nx + A.v * ny;
-      o.u = A.u - 2.0 * un * nx / ds;
-      o.v = A.v - 2.0 * un * ny / ds;
-      o.T = o.p / o.r / ph.R;
+      o.u = A.u - 2.0 * un * nx * ids;
+      o.v = A.v - 2.0 * un * ny * ids;
+      o.T = sdiv(o.p, o.r) * ph.iR;
       o.n = 1.5 * A.n - 0.5 * B.n;
     } break;
     case OP_WALL:
@@ -81,12 +82,12 @@ __device__ __forceinline__ Prim<S> apply_op(const BcNode& nd, const Prim<S>& A, 
       o.u = 2.0 * nd.c0 - (1.5 * A.u - 0.5 * B.u);
       o.v = 2.0 * nd.c1 - (1.5 * A.v - 0.5 * B.v);
       o.T = S(nd.c2);
-      o.r = o.p / o.T / ph.R;
+      o.r = sdiv(o.p, o.T) * ph.iR;
       o.n = -(1.5 * A.n - 0.5 * B.n);
       break;
     case OP_OUTFLOW:
       o.r = A.r; o.u = A.u; o.v = A.v; o.p = S(ph.p_inf);
-      o.T = o.p / o.r / ph.R; o.n = A.n;
+      o.T = sdiv(o.p, o.r) * ph.iR; o.n = A.n;
       break;
     default:  // OP_COPY
       o = A;
@@ -145,7 +146,7 @@ __device__ __forceinline__ S muscl_left(S cm, S c, S cp, double eps) {
   S dm = c - cm, dp = cp - c;
   S a1 = 3.0 * dp * dm;
   S a2 = 2.0 * (dp - dm) * (dp - dm) + a1;
-  S s = 0.25 * (a1 + eps) / (a2 + eps);
+  S s = 0.25 * sdiv(a1 + eps, a2 + eps);
   return c + s * (thm * dm + thp * dp);
 }
 template <class S>
@@ -154,7 +155,7 @@ __device__ __forceinline__ S muscl_right(S cm, S c, S cp, double eps) {
   S dm = c - cm, dp = cp - c;
   S a1 = 3.0 * dp * dm;
   S a2 = 2.0 * (dp - dm) * (dp - dm) + a1;
-  S s = 0.25 * (a1 + eps) / (a2 + eps);
+  S s = 0.25 * sdiv(a1 + eps, a2 + eps);
   return c - s * (thp * dm + thm * dp);
 }
 
@@ -164,17 +165,17 @@ template <class S, bool SA>
 __device__ __forceinline__ void roe_flux(double nx, double ny, S rl, S ul, S vl, S pl, S rr, S ur, S vr, S pr, S ntl,
                                          S ntr, S* f) {
   const double gm1 = kGamma - 1.0, ogm1 = 1.0 / gm1;
-  S rli = 1.0 / rl, rul = rl * ul, rvl = rl * vl, uvl = 0.5 * (ul * ul + vl * vl);
+  S rli = srcp(rl), rul = rl * ul, rvl = rl * vl, uvl = 0.5 * (ul * ul + vl * vl);
   S el = pl * ogm1 + rl * uvl, hl = (el + pl) * rli;
-  S rri = 1.0 / rr, rur = rr * ur, rvr = rr * vr, uvr = 0.5 * (ur * ur + vr * vr);
+  S rri = srcp(rr), rur = rr * ur, rvr = rr * vr, uvr = 0.5 * (ur * ur + vr * vr);
   S er = pr * ogm1 + rr * uvr, hr = (er + pr) * rri;
-  S rat = ssqrt(rr * rli), rati = 1.0 / (rat + 1.0);
+  S rat = ssqrt(rr * rli), rati = srcp(rat + 1.0);
   S rav = rat * rl, uav = (rat * ur + ul) * rati, vav = (rat * vr + vl) * rati, hav = (rat * hr + hl) * rati;
   S uv = 0.5 * (uav * uav + vav * vav);
   S cav = ssqrt(gm1 * (hav - uv));
   S d1 = rr - rl, d2 = ur - ul, d3 = vr - vl, d4 = pr - pl;
-  const double dr = sqrt(nx * nx + ny * ny), r1 = nx / dr, r2 = ny / dr;
-  S uu = r1 * uav + r2 * vav, c2 = cav * cav, c2i = 1.0 / c2;
+  const double dr = sqrt(nx * nx + ny * ny), idr = frcp(dr), r1 = nx * idr, r2 = ny * idr;
+  S uu = r1 * uav + r2 * vav, c2 = cav * cav, c2i = srcp(c2);
   S auu = sabs(uu), aupc = sabs(uu + cav), aumc = sabs(uu - cav);
   S uul = r1 * ul + r2 * vl, uur = r1 * ur + r2 * vr;
   S rcav = rav * cav, aquu = uur - uul, c2ih = 0.5 * c2i, ruuav = auu * rav;
@@ -196,15 +197,15 @@ __device__ __forceinline__ void roe_flux(double nx, double ny, S rl, S ul, S vl,
 
 template <class S>
 struct FaceGeo {
-  double tx, ty, bx, by, lx, ly, rx, ry, vol;  // top/bottom/left/right normals, auxiliary volume
+  double tx, ty, bx, by, lx, ly, rx, ry, ivol;  // top/bottom/left/right normals, 1/auxiliary volume
 };
 
 // Green-Gauss face gradient (src/utils/mesh.cpp:34-131) of one variable from
 // the face's left/right cells and its top/bottom 4-cell averages.
 template <class S>
 __device__ __forceinline__ void gg_grad(const FaceGeo<S>& G, S qt, S qb, S ql, S qr, S* out) {
-  out[0] = (G.tx * qt - G.bx * qb + G.rx * qr - G.lx * ql) / G.vol;
-  out[1] = (G.ty * qt - G.by * qb + G.ry * qr - G.ly * ql) / G.vol;
+  out[0] = (G.tx * qt - G.bx * qb + G.rx * qr - G.lx * ql) * G.ivol;
+  out[1] = (G.ty * qt - G.by * qb + G.ry * qr - G.ly * ql) * G.ivol;
 }
 
 // DiffusiveFluxGreenGauss::evaluate, src/model/flux.cpp:12-48 (+ SA nu~ diffusion).
@@ -223,19 +224,19 @@ __device__ __forceinline__ void visc_flux(double nx, double ny, const S* gu, con
 
 template <class S, bool SA>
 __device__ __forceinline__ void transport(const Prim<S>& P, const DevPhys& ph, S* slot) {
-  S ml = ph.mu_inf * spow23(P.T / ph.T_inf);
+  S ml = ph.mu_inf * spow23(P.T * ph.iTinf);
   if (!SA) {
     slot[L_MU] = ml;
-    slot[L_K] = ml * ph.cp / ph.pr;
+    slot[L_K] = ml * ph.cp * ph.ipr;
     return;
   }
   S rn = P.r * P.n;
-  S chi = rn / ml, chi3 = chi * chi * chi;
+  S chi = sdiv(rn, ml), chi3 = chi * chi * chi;
   S mut(0.0), fn(1.0);
-  if (val(P.n) >= 0.0) mut = rn * (chi3 / (chi3 + kCv1 * kCv1 * kCv1));
-  else fn = (kCn1 + chi3) / (kCn1 - chi3);
+  if (val(P.n) >= 0.0) mut = rn * sdiv(chi3, chi3 + kCv1 * kCv1 * kCv1);
+  else fn = sdiv(kCn1 + chi3, kCn1 - chi3);
   slot[L_MU] = ml + mut;
-  slot[L_K] = ph.cp * (ml / ph.pr + mut / kPrt);
+  slot[L_K] = ph.cp * (ml * ph.ipr + mut * (1.0 / kPrt));
   slot[L_ETA] = ml + rn * fn;
   slot[L_ML] = ml;
 }
@@ -247,29 +248,29 @@ template <class S>
 __device__ __forceinline__ S sa_source(S r, S n, S ml, const S* gu, const S* gv, const S* gn, const S* gr, double d,
                                        double b) {
   const double cw1 = kCb1 / (kKappa * kKappa) + (1.0 + kCb2) / kSigma;
-  S nu = ml / r;
+  S nu = sdiv(ml, r);
   S omega = sabs(gv[0] - gu[1]);
-  const double k2d2 = kKappa * kKappa * d * d;
+  const double k2d2 = kKappa * kKappa * d * d, ik2d2 = frcp(k2d2), id = frcp(d);
   S P, D;
   if (val(n) >= 0.0) {
-    S chi = n / nu, chi3 = chi * chi * chi;
-    S fv1 = chi3 / (chi3 + kCv1 * kCv1 * kCv1);
-    S fv2 = 1.0 - chi / (1.0 + chi * fv1);
-    S sbar = n * fv2 / k2d2;
+    S chi = sdiv(n, nu), chi3 = chi * chi * chi;
+    S fv1 = sdiv(chi3, chi3 + kCv1 * kCv1 * kCv1);
+    S fv2 = 1.0 - sdiv(chi, 1.0 + chi * fv1);
+    S sbar = n * fv2 * ik2d2;
     S stil = (val(sbar) >= -kCv2 * val(omega))
                  ? omega + sbar
-                 : omega + omega * (kCv2 * kCv2 * omega + kCv3 * sbar) / ((kCv3 - 2.0 * kCv2) * omega - sbar);
-    S rr = (val(stil) > 0.0) ? smin(n / (stil * k2d2), 10.0) : S(10.0);
+                 : omega + sdiv(omega * (kCv2 * kCv2 * omega + kCv3 * sbar), (kCv3 - 2.0 * kCv2) * omega - sbar);
+    S rr = (val(stil) > 0.0) ? smin(sdiv(n, stil * k2d2), 10.0) : S(10.0);
     S r2 = rr * rr, r6 = r2 * r2 * r2;
     S gg = rr + kCw2 * (r6 - rr);
     S g2 = gg * gg, g6 = g2 * g2 * g2;
     const double cw3_6 = kCw3 * kCw3 * kCw3 * kCw3 * kCw3 * kCw3;
-    S fw = gg * spow((1.0 + cw3_6) / (g6 + cw3_6), 1.0 / 6.0);
+    S fw = gg * spow(sdiv(S(1.0 + cw3_6), g6 + cw3_6), 1.0 / 6.0);
     P = b * kCb1 * stil * r * n;
-    D = cw1 * fw * r * (n / d) * (n / d);
+    D = cw1 * fw * r * (n * id) * (n * id);
   } else {
     P = b * kCb1 * (1.0 - kCt3) * omega * r * n;
-    D = -cw1 * r * (n / d) * (n / d);
+    D = -cw1 * r * (n * id) * (n * id);
   }
   S cb2t = (kCb2 / kSigma) * r * (gn[0] * gn[0] + gn[1] * gn[1]);
   S cross = -(1.0 / kSigma) * (nu + n) * (gr[0] * gn[0] + gr[1] * gn[1]);
@@ -354,18 +355,18 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
         if (fi == 0) {
           G.tx = nex(0, cj + 1); G.ty = ney(0, cj + 1); G.bx = nex(0, cj); G.by = ney(0, cj);
           G.rx = 0.5 * (ncx(0, cj) + ncx(1, cj)); G.ry = 0.5 * (ncy(0, cj) + ncy(1, cj));
-          G.lx = ncx(0, cj); G.ly = ncy(0, cj); G.vol = vol(0, cj);
+          G.lx = ncx(0, cj); G.ly = ncy(0, cj); G.ivol = frcp(vol(0, cj));
         } else if (fi == nic) {
           G.tx = nex(nic - 1, cj + 1); G.ty = ney(nic - 1, cj + 1); G.bx = nex(nic - 1, cj); G.by = ney(nic - 1, cj);
           G.rx = ncx(nic, cj); G.ry = ncy(nic, cj);
           G.lx = 0.5 * (ncx(nic, cj) + ncx(nic - 1, cj)); G.ly = 0.5 * (ncy(nic, cj) + ncy(nic - 1, cj));
-          G.vol = vol(nic - 1, cj);
+          G.ivol = frcp(vol(nic - 1, cj));
         } else {
           G.tx = 0.5 * (nex(fi, cj + 1) + nex(fi - 1, cj + 1)); G.ty = 0.5 * (ney(fi, cj + 1) + ney(fi - 1, cj + 1));
           G.bx = 0.5 * (nex(fi, cj) + nex(fi - 1, cj)); G.by = 0.5 * (ney(fi, cj) + ney(fi - 1, cj));
           G.rx = 0.5 * (ncx(fi, cj) + ncx(fi + 1, cj)); G.ry = 0.5 * (ncy(fi, cj) + ncy(fi + 1, cj));
           G.lx = 0.5 * (ncx(fi, cj) + ncx(fi - 1, cj)); G.ly = 0.5 * (ncy(fi, cj) + ncy(fi - 1, cj));
-          G.vol = 0.5 * (vol(fi, cj) + vol(fi - 1, cj));
+          G.ivol = frcp(0.5 * (vol(fi, cj) + vol(fi - 1, cj)));
         }
         // left/right cells and their north/south neighbours
         auto grad_avg = [&](int k, S* gr, S* avg) {
@@ -417,18 +418,18 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
           G.tx = 0.5 * (nex(ci, 0) + nex(ci, 1)); G.ty = 0.5 * (ney(ci, 0) + ney(ci, 1));
           G.bx = nex(ci, 0); G.by = ney(ci, 0);
           G.lx = ncx(ci, 0); G.ly = ncy(ci, 0); G.rx = ncx(ci + 1, 0); G.ry = ncy(ci + 1, 0);
-          G.vol = vol(ci, 0);
+          G.ivol = frcp(vol(ci, 0));
         } else if (fj == njc) {
           G.tx = nex(ci, njc); G.ty = ney(ci, njc);
           G.bx = 0.5 * (nex(ci, njc) + nex(ci, njc - 1)); G.by = 0.5 * (ney(ci, njc) + ney(ci, njc - 1));
           G.lx = ncx(ci, njc - 1); G.ly = ncy(ci, njc - 1); G.rx = ncx(ci + 1, njc - 1); G.ry = ncy(ci + 1, njc - 1);
-          G.vol = vol(ci, njc - 1);
+          G.ivol = frcp(vol(ci, njc - 1));
         } else {
           G.tx = 0.5 * (nex(ci, fj) + nex(ci, fj + 1)); G.ty = 0.5 * (ney(ci, fj) + ney(ci, fj + 1));
           G.bx = 0.5 * (nex(ci, fj) + nex(ci, fj - 1)); G.by = 0.5 * (ney(ci, fj) + ney(ci, fj - 1));
           G.lx = 0.5 * (ncx(ci, fj) + ncx(ci, fj - 1)); G.ly = 0.5 * (ncy(ci, fj) + ncy(ci, fj - 1));
           G.rx = 0.5 * (ncx(ci + 1, fj) + ncx(ci + 1, fj - 1)); G.ry = 0.5 * (ncy(ci + 1, fj) + ncy(ci + 1, fj - 1));
-          G.vol = 0.5 * (vol(ci, fj) + vol(ci, fj - 1));
+          G.ivol = frcp(0.5 * (vol(ci, fj) + vol(ci, fj - 1)));
         }
         auto grad_avg = [&](int k, S* gr, S* avg) {
           S t = at(k, x, yn), b = at(k, x, ys);
@@ -460,13 +461,13 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
     const int x = tx + 2, y = ty + 2;
     const double ex = ncx(ci + 1, cj), ey = ncy(ci + 1, cj), wx = ncx(ci, cj), wy = ncy(ci, cj);
     const double nx_ = nex(ci, cj + 1), ny_ = ney(ci, cj + 1), sx = nex(ci, cj), sy = ney(ci, cj);
-    const double V = vol(ci, cj);
+    const double iV = frcp(vol(ci, cj));
     auto cgrad = [&](int k, S* o) {
       S c = at(k, x, y);
       S fe = 0.5 * (c + at(k, x + 1, y)), fw = 0.5 * (c + at(k, x - 1, y));
       S fn = 0.5 * (c + at(k, x, y + 1)), fs = 0.5 * (c + at(k, x, y - 1));
-      o[0] = (fe * ex - fw * wx + fn * nx_ - fs * sx) / V;
-      o[1] = (fe * ey - fw * wy + fn * ny_ - fs * sy) / V;
+      o[0] = (fe * ex - fw * wx + fn * nx_ - fs * sx) * iV;
+      o[1] = (fe * ey - fw * wy + fn * ny_ - fs * sy) * iV;
     };
     S gu[2], gv[2], gn[2], gr[2];
     cgrad(L_U, gu); cgrad(L_V, gv); cgrad(L_N, gn); cgrad(L_R, gr);
@@ -487,7 +488,7 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
 #pragma unroll
   for (int k = 0; k < NV; ++k) sq[k] = 0.0;
   if (owns) {
-    const double V = vol(ci, cj);
+    const double V = vol(ci, cj), iV = frcp(V);
     const size_t cidx = (size_t)cj * g.ldi + ci;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
       if (k == 1) r = r + (-ph.dpdx * V);
       if (k == 2) r = r + (-ph.dpdy * V);
       if (SA && k == 4) r = r + V * src;
-      r = r / V;
+      r = r * iV;
       out.put(k * g.plane + cidx, r);
       sq[k] = val(r) * val(r);
     }

@@ -35,6 +35,49 @@ __device__ __forceinline__ Dual operator/(double a, Dual b) {
 __device__ __forceinline__ Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
 __device__ __forceinline__ Dual& operator-=(Dual& a, Dual b) { a = a - b; return a; }
 
+// Reciprocal from the hardware estimate refined by two Newton steps
+// (r <- r + r (1 - x r)); within ~1 ulp for the normal, finite operands of
+// this solver.  Used in place of IEEE division on the hot path.
+__device__ __forceinline__ double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ double srcp(double x) { return frcp(x); }
+__device__ __forceinline__ Dual srcp(Dual x) {
+  double r = frcp(x.v);
+  return Dual(r, -r * r * x.d);
+}
+__device__ __forceinline__ double sdiv(double a, double b) { return a * frcp(b); }
+__device__ __forceinline__ Dual sdiv(Dual a, Dual b) {
+  double ib = frcp(b.v), r = a.v * ib;
+  return Dual(r, (a.d - r * b.d) * ib);
+}
+__device__ __forceinline__ Dual sdiv(Dual a, double b) {
+  double ib = frcp(b);
+  return Dual(a.v * ib, a.d * ib);
+}
+__device__ __forceinline__ Dual sdiv(double a, Dual b) {
+  double ib = frcp(b.v), r = a * ib;
+  return Dual(r, -r * b.d * ib);
+}
+
+// x^(2/3) for x > 0 without pow: z = x^(-2/3) solves z^-3 = x^2; seed from
+// the single-precision log2/exp2 units, then two Newton steps
+// z <- z (4 - x^2 z^3) / 3 (relative error 2e^2 per step), and
+// x^(2/3) = x^2 z^2.
+__device__ __forceinline__ double pow23(double x) {
+  const float xf = (float)x;
+  const float zf = __builtin_amdgcn_exp2f(-(2.0f / 3.0f) * __builtin_amdgcn_logf(xf));
+  const double a = x * x;
+  double z = (double)zf;
+  z = z * (4.0 - a * (z * z * z)) * (1.0 / 3.0);
+  z = z * (4.0 - a * (z * z * z)) * (1.0 / 3.0);
+  return a * z * z;
+}
+
 __device__ __forceinline__ double val(double x) { return x; }
 __device__ __forceinline__ double val(Dual x) { return x.v; }
 __device__ __forceinline__ double tan_(double) { return 0.0; }
@@ -48,10 +91,10 @@ __device__ __forceinline__ Dual ssqrt(Dual x) {
 __device__ __forceinline__ double sabs(double x) { return fabs(x); }
 __device__ __forceinline__ Dual sabs(Dual x) { return x.v < 0.0 ? -x : x; }
 // x^(2/3) for the power-law viscosity (src/model/fluid.cpp:38-40).
-__device__ __forceinline__ double spow23(double x) { return pow(x, 2.0 / 3.0); }
+__device__ __forceinline__ double spow23(double x) { return pow23(x); }
 __device__ __forceinline__ Dual spow23(Dual x) {
-  double r = pow(x.v, 2.0 / 3.0);
-  return Dual(r, (2.0 / 3.0) * r / x.v * x.d);
+  double r = pow23(x.v);
+  return Dual(r, (2.0 / 3.0) * r * frcp(x.v) * x.d);
 }
 __device__ __forceinline__ double spow(double x, double e) { return pow(x, e); }
 __device__ __forceinline__ Dual spow(Dual x, double e) {

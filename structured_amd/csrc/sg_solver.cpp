@@ -1,0 +1,426 @@
+// Matrix-free Newton-Krylov driver.
+//
+// LinearSolverGMRES keeps the reference's linear-solver interface
+// (preallocate / set_lhs / set_rhs / solve / solve_and_update,
+// src/linearsolver/ls_eigen.h:25-31) but never forms the matrix: set_lhs
+// stores the linearisation state q and the local time steps, and the
+// operator is A z = z/dt - J(q) z with J z from the tangent kernel.  The
+// preconditioner is built from the exact Jacobian blocks J[r, r] and
+// J[r, r -/+ jhat], extracted by a 5x5 cell colouring of J.v products, and
+// factored per i-column as a block-tridiagonal j-line system (or its block
+// diagonal).  The Krylov solver is right-preconditioned restarted FGMRES with
+// classical Gram-Schmidt and one re-orthogonalisation (CGS2).
+//
+// sg_solve runs the pseudo-transient Newton loop of Solver::step/solve
+// (src/solver/solver.cpp:52-236): dt from the CFL, residual and norms,
+// Jacobian, linear solve, q += UR dq, then the CFL and under-relaxation
+// ramps.  Unlike the reference, whose tolerance test is disabled (`&& 0`,
+// solver.cpp:145), the loop stops when every equation's L2 norm is below
+// the tolerance.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "sg_context.hpp"
+#include "sg_kernels.hpp"
+#include "sg_linalg.hpp"
+
+using namespace sg;
+
+#define SG_TRY(expr, what)                         \
+  do {                                             \
+    int rc_ = sg_hip(ctx, (expr), what);           \
+    if (rc_ != SG_OK) return rc_;                  \
+  } while (0)
+
+struct SolverState {
+  int m = 0;                 // restart length
+  size_t n = 0;              // vector length (nvar * plane)
+  DevBuf V, Z, blocks, fac, ilu, idt, dtv, rhs, x, tmp, jz, qop, seed, hdev, ydev, partial, gst;
+  bool have_lhs = false, have_rhs = false;
+  int precond = 4;           // 0 none, 1 block Jacobi, 2 j-line, 3 line Gauss-Seidel, 4 multicolour ILU(0)
+  int sweeps = 1;
+  double rtol = 1e-3;
+  int maxit = 60;
+  int last_iters = 0;
+  double last_relres = 0.0;
+  std::vector<double> l2_hist, cfl_hist;
+  int nhist = 0;
+};
+
+void sg_solver_free(sg_ctx* ctx) {
+  delete ctx->solver;
+  ctx->solver = nullptr;
+}
+
+namespace {
+
+int ensure(sg_ctx* ctx, int restart) {
+  SolverState*& S = ctx->solver;
+  if (!S) S = new SolverState();
+  const size_t n = ctx->nsoa();
+  const int nv = ctx->nvar;
+  if (S->n == n && S->m == restart) return SG_OK;
+  S->n = n;
+  S->m = restart;
+  const size_t nb = ctx->g.plane * NSLOT * nv * nv, nf = ctx->g.plane * 3 * nv * nv;
+  if (S->V.alloc((size_t)(restart + 1) * n) || S->Z.alloc((size_t)restart * n) || S->blocks.alloc(nb) ||
+      S->fac.alloc(nf) || S->ilu.alloc(nb) || S->gst.alloc(n) || S->idt.alloc(ctx->g.plane) || S->dtv.alloc(n) || S->rhs.alloc(n) || S->x.alloc(n) ||
+      S->tmp.alloc(n) || S->jz.alloc(n) || S->qop.alloc(n) || S->seed.alloc(n) || S->hdev.alloc(restart + 2) ||
+      S->ydev.alloc(restart + 2) || S->partial.alloc(multi_dot_partial_size(restart + 2)))
+    return sg_fail(ctx, SG_ERR_HIP, "solver allocation failed");
+  for (DevBuf* b : {&S->V, &S->Z, &S->blocks, &S->fac, &S->dtv, &S->rhs, &S->x, &S->tmp, &S->jz, &S->qop, &S->seed})
+    if (hipMemset(b->p, 0, b->n * sizeof(double)) != hipSuccess) return sg_fail(ctx, SG_ERR_HIP, "memset");
+  return SG_OK;
+}
+
+double host_dot(sg_ctx* ctx, const double* a, const double* b, int* rc) {
+  SolverState* S = ctx->solver;
+  double out = 0.0;
+  hipError_t e = multi_dot(a, 0, 1, b, S->n, S->partial.p, S->hdev.p, false, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&out, S->hdev.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  *rc = sg_hip(ctx, e, "dot");
+  return out;
+}
+
+// z = M^-1 v
+int precond_apply(sg_ctx* ctx, const double* v, double* z) {
+  SolverState* S = ctx->solver;
+  if (S->precond == 0) {
+    SG_TRY(hipMemcpyAsync(z, v, S->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+    return SG_OK;
+  }
+  const int nic = ctx->geo.nic, njc = ctx->geo.njc, ldi = ctx->g.ldi, nv = ctx->nvar;
+  if (S->precond <= 2) {
+    SG_TRY(line_solve(nic, njc, ldi, nv, S->fac.p, v, z, ctx->stream), "precond");
+    return SG_OK;
+  }
+  if (S->precond == 4) {
+    SG_TRY(ilu_apply(nic, njc, ldi, nv, S->ilu.p, v, z, ctx->stream), "ilu apply");
+    return SG_OK;
+  }
+  // symmetric multicolour line Gauss-Seidel from z = 0
+  SG_TRY(hipMemsetAsync(z, 0, S->n * sizeof(double), ctx->stream), "memset");
+  static const int order[6] = {0, 1, 2, 2, 1, 0};
+  for (int sw = 0; sw < S->sweeps; ++sw)
+    for (int c = 0; c < 6; ++c)
+      SG_TRY(line_gs(nic, njc, ldi, nv, order[c], S->blocks.p, S->fac.p, v, S->gst.p, z, ctx->stream), "line gs");
+  return SG_OK;
+}
+
+// w = A z = z/dt - J(qop) z
+int apply_operator(sg_ctx* ctx, const double* z, double* w) {
+  SolverState* S = ctx->solver;
+  SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, S->qop.p, z, nullptr, S->jz.p, ctx->stream), "jv");
+  SG_TRY(shift_minus(z, S->idt.p, S->jz.p, w, ctx->g.plane, ctx->nvar, ctx->stream), "shift");
+  return SG_OK;
+}
+
+int build_preconditioner(sg_ctx* ctx) {
+  SolverState* S = ctx->solver;
+  if (S->precond == 0) return SG_OK;
+  const int nic = ctx->geo.nic, njc = ctx->geo.njc, ldi = ctx->g.ldi, nv = ctx->nvar;
+  for (int a = 0; a < 5; ++a)
+    for (int b = 0; b < 5; ++b)
+      for (int m = 0; m < nv; ++m) {
+        SG_TRY(seed_colour(nic, njc, ldi, nv, a, b, m, S->seed.p, ctx->stream), "seed");
+        SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, S->qop.p, S->seed.p, nullptr, S->jz.p, ctx->stream), "jv");
+        SG_TRY(extract_blocks(nic, njc, ldi, nv, a, b, m, S->jz.p, S->blocks.p, ctx->stream), "extract");
+      }
+  if (S->precond == 4) {
+    SG_TRY(ilu_factor(nic, njc, ldi, nv, S->idt.p, S->blocks.p, S->ilu.p, ctx->stream), "ilu factor");
+    return SG_OK;
+  }
+  SG_TRY(line_factor(nic, njc, ldi, nv, S->idt.p, S->blocks.p, S->fac.p, S->precond == 1, ctx->stream), "factor");
+  return SG_OK;
+}
+
+// Right-preconditioned restarted FGMRES with CGS2.  Solves A x = rhs.
+int fgmres(sg_ctx* ctx, double rtol, int maxit, int* iters, double* relres) {
+  SolverState* S = ctx->solver;
+  const int m = S->m;
+  const size_t n = S->n;
+  int rc = SG_OK;
+  SG_TRY(hipMemsetAsync(S->x.p, 0, n * sizeof(double), ctx->stream), "memset");
+  const double bnorm = std::sqrt(host_dot(ctx, S->rhs.p, S->rhs.p, &rc));
+  if (rc) return rc;
+  *iters = 0;
+  *relres = 0.0;
+  if (bnorm == 0.0) return SG_OK;
+  if (!std::isfinite(bnorm)) return sg_fail(ctx, SG_ERR_NAN, "non-finite right-hand side");
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), hcol(m + 2);
+  int total = 0;
+  double res = bnorm;
+  while (total < maxit) {
+    double* V0 = S->V.p;
+    if (total == 0) {
+      SG_TRY(hipMemcpyAsync(V0, S->rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+    } else {
+      if ((rc = apply_operator(ctx, S->x.p, S->tmp.p))) return rc;
+      SG_TRY(hipMemcpyAsync(V0, S->rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+      SG_TRY(axpby(-1.0, S->tmp.p, 1.0, V0, n, ctx->stream), "axpby");
+    }
+    const double beta = std::sqrt(host_dot(ctx, V0, V0, &rc));
+    if (rc) return rc;
+    if (beta == 0.0) { res = 0.0; break; }
+    SG_TRY(axpby(0.0, V0, 1.0 / beta, V0, n, ctx->stream), "scale");
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int jend = -1;
+    bool done = false;
+    for (int j = 0; j < m && total < maxit; ++j) {
+      double* Vj = S->V.p + (size_t)j * n;
+      double* Zj = S->Z.p + (size_t)j * n;
+      double* W = S->V.p + (size_t)(j + 1) * n;
+      if ((rc = precond_apply(ctx, Vj, Zj))) return rc;
+      if ((rc = apply_operator(ctx, Zj, W))) return rc;
+      // CGS2
+      SG_TRY(multi_dot(S->V.p, n, j + 1, W, n, S->partial.p, S->hdev.p, false, ctx->stream), "dot");
+      SG_TRY(multi_axpy(S->V.p, n, j + 1, S->hdev.p, W, n, ctx->stream), "axpy");
+      SG_TRY(multi_dot(S->V.p, n, j + 1, W, n, S->partial.p, S->ydev.p, false, ctx->stream), "dot");
+      SG_TRY(multi_axpy(S->V.p, n, j + 1, S->ydev.p, W, n, ctx->stream), "axpy");
+      SG_TRY(multi_dot(W, 0, 1, W, n, S->partial.p, S->ydev.p + j + 1, false, ctx->stream), "dot");
+      std::vector<double> h2(j + 2);
+      SG_TRY(hipMemcpyAsync(hcol.data(), S->hdev.p, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+      SG_TRY(hipMemcpyAsync(h2.data(), S->ydev.p, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+      SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+      for (int i = 0; i <= j; ++i) hcol[i] += h2[i];
+      const double hn = std::sqrt(std::max(0.0, h2[j + 1]));
+      if (!std::isfinite(hn)) return sg_fail(ctx, SG_ERR_NAN, "non-finite Krylov vector");
+      hcol[j + 1] = hn;
+      if (hn > 0.0) SG_TRY(axpby(0.0, W, 1.0 / hn, W, n, ctx->stream), "scale");
+      for (int i = 0; i < j; ++i) {  // previous rotations
+        const double t = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
+        hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1];
+        hcol[i] = t;
+      }
+      const double den = std::hypot(hcol[j], hcol[j + 1]);
+      cs[j] = den > 0.0 ? hcol[j] / den : 1.0;
+      sn[j] = den > 0.0 ? hcol[j + 1] / den : 0.0;
+      hcol[j] = den;
+      hcol[j + 1] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hcol[i];
+      ++total;
+      jend = j;
+      res = std::fabs(g[j + 1]);
+      if (res <= rtol * bnorm || hn == 0.0) { done = true; break; }
+    }
+    if (jend < 0) break;
+    for (int i = jend; i >= 0; --i) {  // back substitution
+      double s = g[i];
+      for (int k = i + 1; k <= jend; ++k) s -= H[(size_t)i * m + k] * y[k];
+      y[i] = s / H[(size_t)i * m + i];
+    }
+    SG_TRY(hipMemcpyAsync(S->ydev.p, y.data(), (jend + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "H2D");
+    SG_TRY(lincomb(S->Z.p, n, jend + 1, S->ydev.p, S->x.p, n, true, ctx->stream), "lincomb");
+    if (done) break;
+  }
+  SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  *iters = total;
+  *relres = res / bnorm;
+  return SG_OK;
+}
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double s() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+int upload(sg_ctx* ctx, const double* h, double* soa) {
+  const size_t n = ctx->ncell() * ctx->nvar;
+  SG_TRY(hipMemcpyAsync(ctx->aos_a.p, h, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "H2D");
+  SG_TRY(launch_aos_to_soa(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->nvar, ctx->aos_a.p, soa, ctx->stream), "aos->soa");
+  return SG_OK;
+}
+int download(sg_ctx* ctx, const double* soa, double* h) {
+  const size_t n = ctx->ncell() * ctx->nvar;
+  SG_TRY(launch_soa_to_aos(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->nvar, soa, ctx->aos_b.p, ctx->stream), "soa->aos");
+  SG_TRY(hipMemcpyAsync(h, ctx->aos_b.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  return SG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- LinearSolverGMRES: the reference's linear-solver interface ----------
+int sg_ls_preallocate(sg_ctx* ctx, int restart, int precond, int maxit, double rtol) {
+  if (!ctx || restart < 1 || restart > 200 || precond < 0 || precond > 4) return SG_ERR_ARG;
+  int rc = ensure(ctx, restart);
+  if (rc) return rc;
+  ctx->solver->precond = precond;
+  ctx->solver->maxit = maxit > 0 ? maxit : restart;
+  ctx->solver->rtol = rtol > 0 ? rtol : 1e-3;
+  return SG_OK;
+}
+
+// set_lhs: operator I/dt - J(q) at q (AoS) with the local time steps dt (AoS,
+// the same value for every equation of a cell as calc_dt produces).
+int sg_ls_set_lhs(sg_ctx* ctx, const double* q_aos, const double* dt_aos) {
+  if (!ctx || !q_aos || !dt_aos) return SG_ERR_ARG;
+  int rc;
+  if (!ctx->solver && (rc = ensure(ctx, 30))) return rc;
+  SolverState* S = ctx->solver;
+  if ((rc = upload(ctx, q_aos, S->qop.p))) return rc;
+  if ((rc = upload(ctx, dt_aos, S->dtv.p))) return rc;
+  SG_TRY(invert_dt(S->dtv.p, S->idt.p, ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->stream), "invert");
+  if ((rc = build_preconditioner(ctx))) return rc;
+  SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  S->have_lhs = true;
+  return SG_OK;
+}
+
+int sg_ls_set_rhs(sg_ctx* ctx, const double* rhs_aos) {
+  if (!ctx || !rhs_aos || !ctx->solver) return SG_ERR_ARG;
+  int rc;
+  if ((rc = upload(ctx, rhs_aos, ctx->solver->rhs.p))) return rc;
+  ctx->solver->have_rhs = true;
+  return SG_OK;
+}
+
+int sg_ls_solve(sg_ctx* ctx, double* dq_aos, int* iters, double* relres) {
+  if (!ctx || !ctx->solver) return SG_ERR_ARG;
+  SolverState* S = ctx->solver;
+  if (!S->have_lhs || !S->have_rhs) return sg_fail(ctx, SG_ERR_STATE, "set_lhs and set_rhs must precede solve");
+  int it = 0, rc;
+  double rr = 0.0;
+  if ((rc = fgmres(ctx, S->rtol, S->maxit, &it, &rr))) return rc;
+  S->last_iters = it;
+  S->last_relres = rr;
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  if (dq_aos && (rc = download(ctx, S->x.p, dq_aos))) return rc;
+  return rr <= S->rtol ? SG_OK : sg_fail(ctx, SG_ERR_NOCONV, "GMRES did not reach the tolerance");
+}
+
+// solve_and_update: q <- q + UR * dq (src/linearsolver/ls_eigen.cpp:65-71).
+int sg_ls_solve_and_update(sg_ctx* ctx, double* q_aos, double under_relaxation) {
+  if (!ctx || !q_aos) return SG_ERR_ARG;
+  const size_t n = ctx->ncell() * ctx->nvar;
+  std::vector<double> dq(n);
+  int rc = sg_ls_solve(ctx, dq.data(), nullptr, nullptr);
+  if (rc && rc != SG_ERR_NOCONV) return rc;
+  for (size_t k = 0; k < n; ++k) q_aos[k] += dq[k] * under_relaxation;
+  return rc;
+}
+
+// ---- Solver::solve: pseudo-transient Newton-Krylov ----------------------
+int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_report* rep) {
+  if (!ctx || !q_aos || !o) return SG_ERR_ARG;
+  if (o->gmres_restart < 1 || o->iteration_max < 0) return SG_ERR_ARG;
+  int rc;
+  if ((rc = sg_ls_preallocate(ctx, o->gmres_restart, o->precond, o->gmres_maxit, o->gmres_rtol))) return rc;
+  SolverState* S = ctx->solver;
+  const int nv = ctx->nvar;
+  const size_t n = S->n;
+  Timer wall;
+  double t_res = 0.0, t_jac = 0.0, t_lin = 0.0;
+  double* q = ctx->q.p;
+  if ((rc = upload(ctx, q_aos, q))) return rc;
+  double cfl = o->cfl, ur = o->under_relaxation;
+  S->l2_hist.clear();
+  S->cfl_hist.clear();
+  sg_solve_report R{};
+  double l2[8], l2_0[8];
+  int lin_total = 0;
+  double ser_ref = 0.0;
+  bool converged = false;
+  int it = 0;
+  for (;; ++it) {
+    Timer tr;
+    SG_TRY(launch_dt(ctx->g, ctx->ph.mu_inf, nv, q, ctx->dsmin.p, cfl, S->dtv.p, ctx->stream), "dt");
+    SG_TRY(invert_dt(S->dtv.p, S->idt.p, ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->stream), "invert");
+    SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q, S->rhs.p, ctx->partial.p, nullptr, ctx->l2.p,
+                           ctx->stream), "residual");
+    SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    t_res += tr.s();
+    if (it == 0) std::copy(l2, l2 + nv, l2_0);
+    for (int k = 0; k < nv; ++k) S->l2_hist.push_back(l2[k]);
+    S->cfl_hist.push_back(cfl);
+    bool finite = true;
+    double lmax = 0.0;
+    for (int k = 0; k < nv; ++k) { finite = finite && std::isfinite(l2[k]); lmax = std::max(lmax, l2[k]); }
+    if (o->verbose)
+      std::fprintf(stderr, "newton %4d  cfl %.3e  |R| %.3e %.3e %.3e %.3e %.3e  lin %d (%.1e)\n", it, cfl, l2[0], l2[1],
+                   l2[2], l2[3], nv > 4 ? l2[4] : 0.0, S->last_iters, S->last_relres);
+    if (!finite) { rc = sg_fail(ctx, SG_ERR_NAN, "non-finite residual in the Newton loop"); break; }
+    if (lmax < o->tolerance) { converged = true; break; }
+    if (it >= o->iteration_max) break;
+    Timer tj;
+    SG_TRY(hipMemcpyAsync(S->qop.p, q, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+    if ((rc = build_preconditioner(ctx))) return rc;
+    SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    t_jac += tj.s();
+    Timer tl;
+    int li = 0;
+    double rr = 0.0;
+    if ((rc = fgmres(ctx, S->rtol, S->maxit, &li, &rr))) return rc;
+    S->last_iters = li;
+    S->last_relres = rr;
+    lin_total += li;
+    SG_TRY(axpby(ur, S->x.p, 1.0, q, n, ctx->stream), "update");
+    t_lin += tl.s();
+    // ramps, src/solver/solver.cpp:212-230 (counter = it + 1 after the step)
+    const int counter = it + 1;
+    if (o->cfl_ramp && counter > o->cfl_ramp_iteration) cfl = std::min(std::pow(cfl, o->cfl_ramp_exponent), o->cfl_max);
+    if (o->ser > 0) {  // switched evolution relaxation on the total residual norm
+      double tot = 0.0;
+      for (int k = 0; k < nv; ++k) tot += l2[k] * l2[k];
+      tot = std::sqrt(tot);
+      ser_ref = std::max(ser_ref, tot);
+      cfl = std::min(o->cfl * std::pow(ser_ref / std::max(tot, 1e-300), o->ser_exponent), o->cfl_max);
+      cfl = std::max(cfl, o->cfl);
+    }
+    if (o->under_relaxation_ramp && counter > o->under_relaxation_ramp_iteration)
+      ur = std::min(std::pow(ur, o->under_relaxation_ramp_exponent), 10.0);
+  }
+  int rc2 = download(ctx, q, q_aos);
+  R.iterations = it;
+  R.converged = converged ? 1 : 0;
+  R.linear_iterations = lin_total;
+  for (int k = 0; k < nv && k < 5; ++k) { R.l2_final[k] = l2[k]; R.l2_initial[k] = l2_0[k]; }
+  R.wall_s = wall.s();
+  R.residual_s = t_res;
+  R.jacobian_s = t_jac;
+  R.linsolve_s = t_lin;
+  R.cfl_final = cfl;
+  if (rep) *rep = R;
+  if (rc) return rc;
+  if (rc2) return rc2;
+  return converged ? SG_OK : SG_ERR_NOCONV;
+}
+
+// z = M^-1 v with the preconditioner of the last set_lhs (diagnostic hook).
+int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos) {
+  if (!ctx || !v_aos || !z_aos || !ctx->solver || !ctx->solver->have_lhs) return SG_ERR_STATE;
+  SolverState* S = ctx->solver;
+  int rc;
+  if ((rc = upload(ctx, v_aos, S->tmp.p))) return rc;
+  if ((rc = precond_apply(ctx, S->tmp.p, S->x.p))) return rc;
+  return download(ctx, S->x.p, z_aos);
+}
+
+int sg_solve_history(sg_ctx* ctx, double* l2_hist, double* cfl_hist, int max_entries) {
+  if (!ctx || !ctx->solver) return SG_ERR_ARG;
+  const SolverState* S = ctx->solver;
+  const int n = (int)S->cfl_hist.size();
+  const int m = std::min(n, max_entries);
+  const int nv = ctx->nvar;
+  for (int i = 0; i < m; ++i) {
+    if (cfl_hist) cfl_hist[i] = S->cfl_hist[i];
+    if (l2_hist)
+      for (int k = 0; k < nv; ++k) l2_hist[i * nv + k] = S->l2_hist[(size_t)i * nv + k];
+  }
+  return n;
+}
+
+}  // extern "C"

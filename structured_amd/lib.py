@@ -187,3 +187,46 @@ class Equation:
 
     def soa_to_aos_dev(self, soa_ptr: int, aos_ptr: int, nvar: int | None = None):
         self._check(load().sg_soa_to_aos_dev(self.h, C.c_void_p(soa_ptr), C.c_void_p(aos_ptr), nvar or self.nvar))
+
+    # ---- Newton-Krylov (Solver::solve) and the linear-solver interface
+    def solve(self, q, **opts):
+        """Pseudo-transient Newton-Krylov from q; returns (q, report dict,
+        history dict).  Raises on NaN; a non-converged run is reported, not
+        raised."""
+        q = _f64(q, self.shape).copy()
+        o = abi.newton_opts(**opts)
+        rep = abi.SolveReport()
+        rc = load().sg_solve(self.h, _p(q), C.byref(o), C.byref(rep))
+        if rc not in (0, -5):
+            self._check(rc)
+        n = load().sg_solve_history(self.h, None, None, 0)
+        l2h = np.zeros((max(n, 1), self.nvar))
+        cfl = np.zeros(max(n, 1))
+        load().sg_solve_history(self.h, _p(l2h), _p(cfl), n)
+        report = {f: getattr(rep, f) for f, _ in abi.SolveReport._fields_}
+        report["l2_initial"] = list(rep.l2_initial)[: self.nvar]
+        report["l2_final"] = list(rep.l2_final)[: self.nvar]
+        return q, report, {"l2": l2h[:n], "cfl": cfl[:n]}
+
+    def ls_preallocate(self, restart=30, precond=2, maxit=60, rtol=1e-3):
+        self._check(load().sg_ls_preallocate(self.h, restart, precond, maxit, C.c_double(rtol)))
+
+    def ls_set_lhs(self, q, dt):
+        self._check(load().sg_ls_set_lhs(self.h, _p(_f64(q, self.shape)), _p(_f64(dt, self.shape))))
+
+    def ls_set_rhs(self, r):
+        self._check(load().sg_ls_set_rhs(self.h, _p(_f64(r, self.shape))))
+
+    def ls_solve(self):
+        dq = np.empty(self.shape)
+        it = C.c_int()
+        rr = C.c_double()
+        rc = load().sg_ls_solve(self.h, _p(dq), C.byref(it), C.byref(rr))
+        if rc not in (0, -5):
+            self._check(rc)
+        return dq, it.value, rr.value
+
+    def ls_precond_apply(self, v):
+        z = np.empty(self.shape)
+        self._check(load().sg_ls_precond_apply(self.h, _p(_f64(v, self.shape)), _p(z)))
+        return z

@@ -1,0 +1,124 @@
+"""numpy restatement of the library's Jacobian-block preconditioners (test
+infrastructure): block Jacobi, j-line block-tridiagonal, and symmetric
+multicolour j-line Gauss-Seidel, built from a dense Jacobian."""
+import numpy as np
+
+SLOTS = [(0, 0), (0, -1), (0, 1), (-1, 0), (1, 0), (-1, -1), (1, -1), (-1, 1), (1, 1), (0, -2), (0, 2), (-2, 0), (2, 0)]
+
+
+def blocks_from_dense(J, nic, njc, nv):
+    """B[i][j][s] = J[(i,j), (i,j)+SLOTS[s]] (nv x nv), zero outside the grid."""
+    Jr = J.reshape(nic, njc, nv, nic, njc, nv)
+    B = np.zeros((nic, njc, len(SLOTS), nv, nv))
+    for i in range(nic):
+        for j in range(njc):
+            for s, (di, dj) in enumerate(SLOTS):
+                ii, jj = i + di, j + dj
+                if 0 <= ii < nic and 0 <= jj < njc:
+                    B[i, j, s] = Jr[i, j, :, ii, jj, :]
+    return B
+
+
+def line_solve(B, idt, i, t, jacobi=False):
+    """Solve the i-th j-line (I/dt - J) restricted to slots 0,1,2 (or 0)."""
+    njc, nv = t.shape
+    A = np.zeros((njc * nv, njc * nv))
+    for j in range(njc):
+        A[j * nv:(j + 1) * nv, j * nv:(j + 1) * nv] = np.eye(nv) * idt[i, j] - B[i, j, 0]
+        if not jacobi:
+            if j > 0:
+                A[j * nv:(j + 1) * nv, (j - 1) * nv:j * nv] = -B[i, j, 1]
+            if j < njc - 1:
+                A[j * nv:(j + 1) * nv, (j + 1) * nv:(j + 2) * nv] = -B[i, j, 2]
+    return np.linalg.solve(A, t.reshape(-1)).reshape(njc, nv)
+
+
+def apply(B, idt, v, precond, sweeps=1):
+    v = np.asarray(v, dtype=np.float64)
+    nic, njc, nv = v.shape
+    if precond == 0:
+        return v.copy()
+    if precond == 4:
+        return ilu_mc5(B, idt, v)
+    if precond in (1, 2):
+        return np.stack([line_solve(B, idt, i, v[i], jacobi=(precond == 1)) for i in range(nic)])
+    z = np.zeros_like(v)
+    for _ in range(sweeps):
+        for colour in (0, 1, 2, 2, 1, 0):
+            for i in range(colour, nic, 3):
+                t = v[i].copy()
+                for j in range(njc):
+                    for s in range(3, len(SLOTS)):
+                        di, dj = SLOTS[s]
+                        ii, jj = i + di, j + dj
+                        if 0 <= ii < nic and 0 <= jj < njc:
+                            t[j] += B[i, j, s] @ z[ii, jj]
+                z[i] = line_solve(B, idt, i, t)
+    return z
+
+
+def colour5(i, j):
+    return (i + 3 * j) % 5
+
+
+def ilu_mc5(B, idt, v):
+    """Five-colour block ILU(0) of I/dt - J on the 13-point stencil applied to
+    v: factor rows colour by colour (lower neighbours eliminated in colour
+    order), then forward/backward solves by colour."""
+    v = np.asarray(v, dtype=np.float64)
+    nic, njc, nv = v.shape
+    A = {}
+    for i in range(nic):
+        for j in range(njc):
+            for s, (di, dj) in enumerate(SLOTS):
+                ii, jj = i + di, j + dj
+                if 0 <= ii < nic and 0 <= jj < njc:
+                    blk = -B[i, j, s].copy()
+                    if s == 0:
+                        blk += np.eye(nv) * idt[i, j]
+                    A[(i, j, ii, jj)] = blk
+    Dinv = {}
+    for c in range(5):
+        for i in range(nic):
+            for j in range(njc):
+                if colour5(i, j) != c:
+                    continue
+                for cn in range(c):
+                    for (di, dj) in SLOTS[1:]:
+                        ni, nj = i + di, j + dj
+                        if not (0 <= ni < nic and 0 <= nj < njc) or colour5(ni, nj) != cn:
+                            continue
+                        L = A[(i, j, ni, nj)] @ Dinv[(ni, nj)]
+                        A[(i, j, ni, nj)] = L
+                        for (ti, tj) in SLOTS[1:]:
+                            ji, jj = ni + ti, nj + tj
+                            if not (0 <= ji < nic and 0 <= jj < njc) or colour5(ji, jj) <= cn:
+                                continue
+                            if (ji - i, jj - j) not in SLOTS:
+                                continue
+                            A[(i, j, ji, jj)] = A[(i, j, ji, jj)] - L @ A[(ni, nj, ji, jj)]
+                Dinv[(i, j)] = np.linalg.inv(A[(i, j, i, j)])
+    z = np.zeros_like(v)
+    for c in range(5):
+        for i in range(nic):
+            for j in range(njc):
+                if colour5(i, j) != c:
+                    continue
+                acc = v[i, j].copy()
+                for (di, dj) in SLOTS[1:]:
+                    ni, nj = i + di, j + dj
+                    if 0 <= ni < nic and 0 <= nj < njc and colour5(ni, nj) < c:
+                        acc -= A[(i, j, ni, nj)] @ z[ni, nj]
+                z[i, j] = acc
+    for c in range(4, -1, -1):
+        for i in range(nic):
+            for j in range(njc):
+                if colour5(i, j) != c:
+                    continue
+                acc = z[i, j].copy()
+                for (di, dj) in SLOTS[1:]:
+                    ni, nj = i + di, j + dj
+                    if 0 <= ni < nic and 0 <= nj < njc and colour5(ni, nj) > c:
+                        acc -= A[(i, j, ni, nj)] @ z[ni, nj]
+                z[i, j] = Dinv[(i, j)] @ acc
+    return z
