@@ -104,8 +104,38 @@ int sg_residual(sg_ctx* ctx, const double* q_aos, double* r_aos, double* l2);
 int sg_jv(sg_ctx* ctx, const double* q_aos, const double* v_aos, double* jv_aos);
 
 /* ---- adjoint: J^T.psi and (dR/dbeta)^T.psi (no counterpart in the
- *      reference, which has no reverse mode; dbeta is [nic][njc]). */
+ *      reference, which has no reverse mode; dbeta is [nic][njc]).  J^T.psi is
+ *      applied in gather form from the exact Jacobian blocks of the 13-point
+ *      stencil (extracted at q by coloured tangent sweeps); dR/dbeta is
+ *      diagonal (beta_c enters only the SA production of cell c) and comes
+ *      from one tangent sweep in the beta direction. */
 int sg_jtv(sg_ctx* ctx, const double* q_aos, const double* psi_aos, double* jtpsi_aos, double* dbeta);
+
+typedef struct sg_adjoint_opts {
+  int restart;     /* GMRES restart length                                  */
+  int maxit;       /* total Krylov iterations                               */
+  double rtol;     /* relative residual tolerance                           */
+  int precond;     /* 0 none, 4 five-colour ILU(0), 5 natural-order ILU(0)  */
+  int reserved;
+  double pc_cfl;   /* the preconditioner factors I/dt - J at this CFL        */
+} sg_adjoint_opts;
+typedef struct sg_adjoint_report {
+  int iterations;
+  int reserved;
+  double relres;
+  double wall_s;
+} sg_adjoint_report;
+
+/* Solve J^T psi = rhs at q (all AoS). */
+int sg_adjoint_solve(sg_ctx* ctx, const double* q_aos, const double* rhs_aos, double* psi_aos,
+                     const sg_adjoint_opts* opts, sg_adjoint_report* report);
+/* Wall objective F = sum_i ds_i (Cf_i - Cf*_i)^2 over the surface cells of
+ * sg_wall (cf_target has nb entries) and its exact gradient dF/dq (AoS, may
+ * be NULL). */
+int sg_objective_cf(sg_ctx* ctx, const double* q_aos, const double* cf_target, double* F, double* dFdq_aos);
+/* dF/dbeta [nic][njc] at a converged state q via one adjoint solve. */
+int sg_adjoint_gradient(sg_ctx* ctx, const double* q_aos, const double* cf_target, double* grad_beta, double* F,
+                        const sg_adjoint_opts* opts, sg_adjoint_report* report);
 
 /* ---- local time step: EulerEquation::calc_dt, src/model/eulerequation.cpp:237-259 */
 int sg_dt(sg_ctx* ctx, const double* q_aos, double cfl, double* dt_aos);
@@ -134,7 +164,8 @@ void* sg_get_stream(sg_ctx* ctx);
  *      also builds the preconditioner from the exact Jacobian's 13 stencil
  *      blocks per cell (precond 0: none, 1: block Jacobi, 2: j-line
  *      block-tridiagonal, 3: symmetric multicolour j-line Gauss-Seidel,
- *      4: five-colour block ILU(0) of I/dt - J on the 13-point stencil). 
+ *      4: five-colour block ILU(0) of I/dt - J on the 13-point stencil,
+ *      5: block ILU(0) in natural j-major order, level-scheduled). 
  *      The Krylov method is restarted FGMRES (restart length `restart`,
  *      at most `maxit` iterations, relative tolerance `rtol`). */
 int sg_ls_preallocate(sg_ctx* ctx, int restart, int precond, int maxit, double rtol);
@@ -170,6 +201,14 @@ typedef struct sg_newton_opts {
   int ser;
   double ser_exponent;
   int verbose;
+  /* > 0: the preconditioner factors I/dt_pc - J with dt_pc taken at
+   * min(cfl, pc_cfl_max) (a diagonal boost that keeps the incomplete
+   * factorisation stable at large CFL); the Krylov operator keeps dt. */
+  double pc_cfl_max;
+  /* 0: stop on the reference's norm ||R_k||_2 (R = flux balance / V); 1: on
+   * ||V R_k||_2, whose roundoff floor does not grow as cells shrink. */
+  int norm_weighted;
+  int reserved2;
 } sg_newton_opts;
 
 typedef struct sg_solve_report {
@@ -179,6 +218,7 @@ typedef struct sg_solve_report {
   int reserved;
   double l2_initial[5];
   double l2_final[5];
+  double l2w_final[5];   /* ||V R_k||_2 at the end */
   double cfl_final;
   double wall_s, residual_s, jacobian_s, linsolve_s;  /* Profiler buckets, src/utils/config.cpp:5-29 */
 } sg_solve_report;

@@ -49,13 +49,14 @@ class NewtonOpts(C.Structure):
                 ("under_relaxation_ramp", C.c_int), ("under_relaxation_ramp_iteration", C.c_int),
                 ("under_relaxation_ramp_exponent", C.c_double), ("iteration_max", C.c_int), ("tolerance", C.c_double),
                 ("gmres_restart", C.c_int), ("gmres_maxit", C.c_int), ("gmres_rtol", C.c_double), ("precond", C.c_int),
-                ("ser", C.c_int), ("ser_exponent", C.c_double), ("verbose", C.c_int)]
+                ("ser", C.c_int), ("ser_exponent", C.c_double), ("verbose", C.c_int), ("pc_cfl_max", C.c_double),
+                ("norm_weighted", C.c_int), ("reserved2", C.c_int)]
 
 
 class SolveReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("linear_iterations", C.c_int),
                 ("reserved", C.c_int), ("l2_initial", C.c_double * 5), ("l2_final", C.c_double * 5),
-                ("cfl_final", C.c_double), ("wall_s", C.c_double), ("residual_s", C.c_double),
+                ("l2w_final", C.c_double * 5), ("cfl_final", C.c_double), ("wall_s", C.c_double), ("residual_s", C.c_double),
                 ("jacobian_s", C.c_double), ("linsolve_s", C.c_double)]
 
 
@@ -64,7 +65,8 @@ class SolveReport(C.Structure):
 NEWTON_DEFAULTS = dict(cfl=1.0, cfl_ramp=0, cfl_ramp_iteration=20, cfl_ramp_exponent=1.1, cfl_max=1e12,
                        under_relaxation=1.0, under_relaxation_ramp=0, under_relaxation_ramp_iteration=20,
                        under_relaxation_ramp_exponent=1.1, iteration_max=1, tolerance=1e-10, gmres_restart=30,
-                       gmres_maxit=60, gmres_rtol=1e-3, precond=2, ser=0, ser_exponent=1.0, verbose=0)
+                       gmres_maxit=60, gmres_rtol=1e-3, precond=4, ser=0, ser_exponent=1.0, verbose=0,
+                       pc_cfl_max=0.0, norm_weighted=0, reserved2=0)
 
 
 def newton_opts(**kw):
@@ -74,3 +76,12 @@ def newton_opts(**kw):
         raise KeyError("unknown Newton options: %s" % sorted(unknown))
     d.update(kw)
     return NewtonOpts(**d)
+
+
+class AdjointOpts(C.Structure):
+    _fields_ = [("restart", C.c_int), ("maxit", C.c_int), ("rtol", C.c_double), ("precond", C.c_int),
+                ("reserved", C.c_int), ("pc_cfl", C.c_double)]
+
+
+class AdjointReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("reserved", C.c_int), ("relres", C.c_double), ("wall_s", C.c_double)]

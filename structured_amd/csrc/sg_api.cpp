@@ -201,6 +201,35 @@ bool BcCompiled::compile(const HostGeom& g, const sg_bc* bcs, int nbc, bool visc
       }
     }
   }
+  // Locality of the ghost dependencies (see BcCompiled::local): every
+  // interior cell that reads a ghost (those within one cell of it) must see
+  // the ghost's interior operands inside its 13-point stencil.
+  local = true;
+  auto in_stencil = [](int di, int dj) {
+    return (std::abs(di) <= 1 && std::abs(dj) <= 1) || (di == 0 && std::abs(dj) == 2) || (dj == 0 && std::abs(di) == 2);
+  };
+  for (int gi = 0; gi <= nic + 1 && local; ++gi)
+    for (int gj = 0; gj <= njc + 1 && local; ++gj) {
+      const int p = C.perim(gi, gj);
+      if (p < 0 || ghost_map[p] < 0) continue;
+      std::vector<int> stack{ghost_map[p]}, leaves;
+      while (!stack.empty()) {
+        const BcNode& nd = nodes[stack.back()];
+        stack.pop_back();
+        if (nd.op == OP_FREESTREAM) continue;
+        const bool two = nd.op == OP_SLIP || nd.op == OP_WALL || nd.op == OP_ISO;
+        for (int r : {nd.ra, two ? nd.rb : nd.ra}) {
+          if (r < 0) stack.push_back(-(r + 1));
+          else leaves.push_back(r);
+        }
+      }
+      for (int ui = gi - 1; ui <= gi + 1 && local; ++ui)
+        for (int uj = gj - 1; uj <= gj + 1 && local; ++uj) {
+          if (ui < 1 || ui > nic || uj < 1 || uj > njc) continue;  // users are interior cells
+          for (int r : leaves)
+            if (!in_stencil(r % nic + 1 - ui, r / nic + 1 - uj)) { local = false; break; }
+        }
+    }
   // Edge ghosts feed the boundary faces; corners feed the viscous stencil.
   for (int gi = 1; gi <= nic; ++gi)
     if (ghost_map[C.perim(gi, 0)] < 0 || ghost_map[C.perim(gi, njc + 1)] < 0) { err = "bottom/top ghost row not fully set by the boundary list"; return false; }
@@ -393,9 +422,9 @@ int sg_create(int ni, int nj, const double* xv, const double* yv, const sg_frees
   }
   // work buffers
   const size_t naos = ctx->ncell() * ctx->nvar, nsoa = ctx->nsoa();
-  const size_t npart = residual_partial_count(g) * ctx->nvar;
+  const size_t npart = residual_partial_count(g) * 2 * ctx->nvar;
   if (ctx->aos_a.alloc(naos) || ctx->aos_b.alloc(naos) || ctx->q.alloc(nsoa) || ctx->r.alloc(nsoa) ||
-      ctx->v.alloc(nsoa) || ctx->w.alloc(nsoa) || ctx->partial.alloc(npart) || ctx->l2.alloc(8) || ctx->counter.alloc(1) ||
+      ctx->v.alloc(nsoa) || ctx->w.alloc(nsoa) || ctx->partial.alloc(npart) || ctx->l2.alloc(16) || ctx->counter.alloc(1) ||
       ctx->wall_cp.alloc(G.ni) || ctx->wall_cf.alloc(G.ni) || ctx->wall_f.alloc(4))
     return fail(SG_ERR_HIP);
   // zero the SoA padding once so padded lanes never hold garbage

@@ -50,6 +50,10 @@ struct BcCompiled {
   std::vector<BcNode> nodes;
   std::vector<int> ghost_map;
   std::string err;
+  // every ghost cell depends only on interior cells within one cell
+  // tangentially and two normally (so every Jacobian coupling stays inside
+  // the 13-point stencil); false for wake / wide periodic boundaries
+  bool local = true;
   bool compile(const HostGeom& g, const sg_bc* bcs, int nbc, bool viscous);
 };
 

@@ -23,20 +23,20 @@ hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc,
   Field<double> fq{q};
   OutField<double> fo{r};
   (void)counter;
-  if (sa) hipLaunchKernelGGL((residual_kernel<double, true>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial);
-  else hipLaunchKernelGGL((residual_kernel<double, false>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial);
+  if (sa) hipLaunchKernelGGL((residual_kernel<double, true>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial, nullptr);
+  else hipLaunchKernelGGL((residual_kernel<double, false>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial, nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !partial || !l2) return e;
-  return launch_norm_finalize(partial, (int)residual_partial_count(g), sa ? 5 : 4, l2, s);
+  return launch_norm_finalize(partial, (int)residual_partial_count(g), 2 * (sa ? 5 : 4), l2, s);
 }
 
 hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,
-                     double* r, double* jv, hipStream_t s) {
+                     double* r, double* jv, hipStream_t s, const double* beta_dot) {
   dim3 grid = tile_grid(g);
   Field<Dual> fq{q, v};
   OutField<Dual> fo{r, jv};
-  if (sa) hipLaunchKernelGGL((residual_kernel<Dual, true>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, nullptr);
-  else hipLaunchKernelGGL((residual_kernel<Dual, false>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, nullptr);
+  if (sa) hipLaunchKernelGGL((residual_kernel<Dual, true>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, nullptr, beta_dot);
+  else hipLaunchKernelGGL((residual_kernel<Dual, false>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, nullptr, nullptr);
   return hipGetLastError();
 }
 
@@ -44,8 +44,8 @@ hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool 
 // fixed butterfly and wave order, so the result is bitwise reproducible),
 // then sqrt.  One block, one pass over the partials.
 __global__ __launch_bounds__(256) void norm_finalize_kernel(const double* partial, int nblocks, int nv, double* l2) {
-  __shared__ double red[4][8];
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  __shared__ double red[4][10];
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nblocks; b += 256)
     for (int k = 0; k < nv; ++k) acc[k] += partial[(size_t)b * nv + k];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -112,24 +112,23 @@ hipError_t launch_soa_to_aos(int nic, int njc, int ldi, int nv, const double* so
 // Green-Gauss gradients of u, v (the reference reads them from the last
 // residual evaluation), Cp from the first two interior cells and Cf from
 // mu_inf (u_y - v_x); the four force sums are reduced in a fixed order.
-template <bool SA>
-__global__ void wall_kernel(DevGeom g, DevPhys ph, DevBc bc, const double* q, int first, int nb, const double* xv0,
-                            const double* yv0, double qinf, double* cp_out, double* cf_out, double* fsum) {
+template <class S, bool SA>
+__global__ void wall_kernel(DevGeom g, DevPhys ph, DevBc bc, Field<S> fq, int first, int nb, const double* xv0,
+                            const double* yv0, double qinf, double* cp_out, double* cf_out, double* dcf_out,
+                            double* fsum) {
   __shared__ double red[4][256];
-  Field<double> fq{q};
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int nic = g.nic;
   for (int n = threadIdx.x; n < nb; n += blockDim.x) {
     const int ci = first + n;
-    auto prim = [&](int a, int b) -> Prim<double> {
-      if (a >= 0 && a < nic && b >= 0 && b < g.njc) return prim_interior<double, SA>(fq, g, ph, a, b);
-      return eval_node<double, SA>(bc.ghost_map[ghost_index(a, b, nic, g.njc)], bc, fq, g, ph);
+    auto prim = [&](int a, int b) -> Prim<S> {
+      if (a >= 0 && a < nic && b >= 0 && b < g.njc) return prim_interior<S, SA>(fq, g, ph, a, b);
+      return eval_node<S, SA>(bc.ghost_map[ghost_index(a, b, nic, g.njc)], bc, fq, g, ph);
     };
-    Prim<double> c0 = prim(ci, 0), c1 = prim(ci, 1);
-    double cp = (0.5 * (c0.p + c1.p) - ph.p_inf) / qinf;
+    Prim<S> c0 = prim(ci, 0), c1 = prim(ci, 1);
+    S cp = (0.5 * (c0.p + c1.p) - ph.p_inf) / qinf;
     // eta face j = 0 of column ci (src/utils/mesh.cpp:99-104,124-125)
-    Prim<double> s0 = prim(ci, -1), w0 = prim(ci - 1, 0), ws = prim(ci - 1, -1), e0 = prim(ci + 1, 0),
-                 es = prim(ci + 1, -1);
+    Prim<S> s0 = prim(ci, -1), w0 = prim(ci - 1, 0), ws = prim(ci - 1, -1), e0 = prim(ci + 1, 0), es = prim(ci + 1, -1);
     auto NX = [&](int a, int b) { return g.nex[(size_t)b * g.ldi + a]; };
     auto NY = [&](int a, int b) { return g.ney[(size_t)b * g.ldi + a]; };
     auto CX = [&](int a, int b) { return g.ncx[(size_t)b * g.ldf + a]; };
@@ -137,25 +136,26 @@ __global__ void wall_kernel(DevGeom g, DevPhys ph, DevBc bc, const double* q, in
     const double tx = 0.5 * (NX(ci, 0) + NX(ci, 1)), ty = 0.5 * (NY(ci, 0) + NY(ci, 1));
     const double bx = NX(ci, 0), by = NY(ci, 0), lx = CX(ci, 0), ly = CY(ci, 0), rx = CX(ci + 1, 0), ry = CY(ci + 1, 0);
     const double V = g.vol[ci];
-    auto grad = [&](double t, double b, double w, double wsv, double e, double esv, double* o) {
-      double l = 0.25 * (t + b + w + wsv), r = 0.25 * (t + b + e + esv);
+    auto grad = [&](S t, S b, S w, S wsv, S e, S esv, S* o) {
+      S l = 0.25 * (t + b + w + wsv), r = 0.25 * (t + b + e + esv);
       o[0] = (tx * t - bx * b + rx * r - lx * l) / V;
       o[1] = (ty * t - by * b + ry * r - ly * l) / V;
     };
-    double gu[2], gv[2];
+    S gu[2], gv[2];
     grad(c0.u, s0.u, w0.u, ws.u, e0.u, es.u, gu);
     grad(c0.v, s0.v, w0.v, ws.v, e0.v, es.v, gv);
-    double tau = ph.mu_inf * (gu[1] - gv[0]) / qinf;
-    cp_out[n] = cp;
-    cf_out[n] = tau;
+    S tau = ph.mu_inf * (gu[1] - gv[0]) / qinf;
+    if (cp_out) cp_out[n] = val(cp);
+    if (cf_out) cf_out[n] = val(tau);
+    if (dcf_out) dcf_out[n] = tan_(tau);
     const double dx = xv0[ci + 1] - xv0[ci], dy = yv0[ci + 1] - yv0[ci];
-    double sfdiv = 2.0 / 3.0 * (gu[0] + gv[1]);
-    double sxx = ph.mu_inf * (2.0 * gu[0] - sfdiv) / qinf;
-    double syy = ph.mu_inf * (2.0 * gv[1] - sfdiv) / qinf;
-    acc[0] += -cp * dx;
-    acc[1] += cp * dy;
-    acc[2] += -tau * dy + syy * dx;
-    acc[3] += tau * dx - sxx * dy;
+    double sfdiv = 2.0 / 3.0 * (val(gu[0]) + val(gv[1]));
+    double sxx = ph.mu_inf * (2.0 * val(gu[0]) - sfdiv) / qinf;
+    double syy = ph.mu_inf * (2.0 * val(gv[1]) - sfdiv) / qinf;
+    acc[0] += -val(cp) * dx;
+    acc[1] += val(cp) * dy;
+    acc[2] += -val(tau) * dy + syy * dx;
+    acc[3] += val(tau) * dx - sxx * dy;
   }
   for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = acc[k];
   __syncthreads();
@@ -164,16 +164,28 @@ __global__ void wall_kernel(DevGeom g, DevPhys ph, DevBc bc, const double* q, in
       for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 4) fsum[threadIdx.x] = red[threadIdx.x][0];
+  if (fsum && threadIdx.x < 4) fsum[threadIdx.x] = red[threadIdx.x][0];
 }
 
 hipError_t launch_wall(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, int first,
                        int nb, const double* xv0, const double* yv0, double qinf, double* cp, double* cf, double* fsum,
                        hipStream_t s) {
-  if (sa) hipLaunchKernelGGL((wall_kernel<true>), dim3(1), dim3(256), 0, s, g, ph, bc, q, first, nb, xv0, yv0, qinf, cp,
-                             cf, fsum);
-  else hipLaunchKernelGGL((wall_kernel<false>), dim3(1), dim3(256), 0, s, g, ph, bc, q, first, nb, xv0, yv0, qinf, cp,
-                          cf, fsum);
+  Field<double> fq{q};
+  if (sa) hipLaunchKernelGGL((wall_kernel<double, true>), dim3(1), dim3(256), 0, s, g, ph, bc, fq, first, nb, xv0, yv0,
+                             qinf, cp, cf, nullptr, fsum);
+  else hipLaunchKernelGGL((wall_kernel<double, false>), dim3(1), dim3(256), 0, s, g, ph, bc, fq, first, nb, xv0, yv0,
+                          qinf, cp, cf, nullptr, fsum);
+  return hipGetLastError();
+}
+
+hipError_t launch_wall_tangent(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q,
+                               const double* v, int first, int nb, const double* xv0, const double* yv0, double qinf,
+                               double* dcf, hipStream_t s) {
+  Field<Dual> fq{q, v};
+  if (sa) hipLaunchKernelGGL((wall_kernel<Dual, true>), dim3(1), dim3(256), 0, s, g, ph, bc, fq, first, nb, xv0, yv0,
+                             qinf, nullptr, nullptr, dcf, nullptr);
+  else hipLaunchKernelGGL((wall_kernel<Dual, false>), dim3(1), dim3(256), 0, s, g, ph, bc, fq, first, nb, xv0, yv0,
+                          qinf, nullptr, nullptr, dcf, nullptr);
   return hipGetLastError();
 }
 

@@ -7,13 +7,14 @@
 namespace sg {
 
 size_t residual_partial_count(const DevGeom& g);
-// partial: per-block norm partials (or null); with l2 (device, nv entries)
-// the last block also finalises the norm.  counter must hold 0 on entry and
-// is left at 0.
+// partial: per-block norm partials (2 nv per block, or null); with l2
+// (device, 2 nv entries: the L2 norms of R, then of V R) a second kernel
+// finalises the norms.  counter is unused.
 hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
                            double* partial, unsigned* counter, double* l2, hipStream_t s);
+// J.v (+ (dR/dbeta).beta_dot when beta_dot is not null; v may be all zero)
 hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,
-                     double* r, double* jv, hipStream_t s);
+                     double* r, double* jv, hipStream_t s, const double* beta_dot = nullptr);
 hipError_t launch_norm_finalize(const double* partial, int nblocks, int nv, double* l2, hipStream_t s);
 hipError_t launch_dt(const DevGeom& g, double mu_inf, int nv, const double* q, const double* dsmin, double cfl,
                      double* dt, hipStream_t s);
@@ -22,5 +23,10 @@ hipError_t launch_soa_to_aos(int nic, int njc, int ldi, int nv, const double* so
 hipError_t launch_wall(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, int first,
                        int nb, const double* xv0, const double* yv0, double qinf, double* cp, double* cf, double* fsum,
                        hipStream_t s);
+
+// tangent of the wall Cf along v (dcf has nb entries)
+hipError_t launch_wall_tangent(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q,
+                               const double* v, int first, int nb, const double* xv0, const double* yv0, double qinf,
+                               double* dcf, hipStream_t s);
 
 }  // namespace sg

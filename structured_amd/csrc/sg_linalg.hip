@@ -469,6 +469,286 @@ hipError_t ilu_apply(int nic, int njc, int ldi, int nv, const double* ilu, const
   return hipGetLastError();
 }
 
+// ---- level-scheduled block ILU(0) in natural j-major order -----------------
+// Cells ordered o = j*nic + i (i fastest).  The lower neighbours of (i,j) on
+// the 13-point stencil are (0,-2), (-1,-1), (0,-1), (1,-1), (-2,0), (-1,0)
+// (listed in increasing o); all of them lie on an earlier level L = i + 2j,
+// all upper neighbours on a later one.  One persistent work-group walks the
+// levels with a barrier between them; the cells of one level run in
+// parallel.  Storage as the multicolour variant (slot 0: inverse of the
+// factored diagonal, lower slots L_rn, upper slots U_rj), SoA.
+constexpr int NLOWER = 6;
+__constant__ int kLowDi[NLOWER] = {0, -1, 0, 1, -2, -1};
+__constant__ int kLowDj[NLOWER] = {-2, -1, -1, -1, 0, 0};
+
+__device__ __forceinline__ bool is_lower(int di, int dj) { return dj < 0 || (dj == 0 && di < 0); }
+
+template <int NV>
+__global__ __launch_bounds__(1024) void lilu_factor_kernel(int nic, int njc, int ldi, double* ilu) {
+  constexpr int B = NV * NV;
+  const size_t plane = (size_t)njc * ldi;
+  const int nlev = nic + 2 * (njc - 1);
+  for (int L = 0; L < nlev; ++L) {
+    const int jlo = max(0, (L - nic + 2) / 2), jhi = min(njc - 1, L / 2);
+    for (int j = jlo + (int)threadIdx.x; j <= jhi; j += blockDim.x) {
+      const int i = L - 2 * j;
+      if (i < 0 || i >= nic) continue;
+      const size_t c = (size_t)j * ldi + i;
+      for (int q = 0; q < NLOWER; ++q) {
+        const int di = kLowDi[q], dj = kLowDj[q];
+        const int ni_ = i + di, nj_ = j + dj;
+        if (ni_ < 0 || ni_ >= nic || nj_ < 0) continue;
+        const size_t cnb = (size_t)nj_ * ldi + ni_;
+        const int s = slot_index(di, dj);
+        double Lb[B];
+        for (int r = 0; r < NV; ++r)
+          for (int t = 0; t < NV; ++t) {
+            double sum = 0.0;
+            for (int k = 0; k < NV; ++k)
+              sum += ilu[(size_t)(s * B + r * NV + k) * plane + c] * ilu[(size_t)(0 * B + k * NV + t) * plane + cnb];
+            Lb[r * NV + t] = sum;
+          }
+        for (int e = 0; e < B; ++e) ilu[(size_t)(s * B + e) * plane + c] = Lb[e];
+        // upper neighbours u of n (o(u) > o(n)) that are also in r's stencil
+        for (int t = 1; t < NSLOT; ++t) {
+          const int ti = kSlotDi[t], tj = kSlotDj[t];
+          if (is_lower(ti, tj)) continue;
+          const int ui = ni_ + ti, uj = nj_ + tj;
+          if (ui < 0 || ui >= nic || uj < 0 || uj >= njc) continue;
+          const int u = slot_index(di + ti, dj + tj);
+          if (u < 0) continue;
+          for (int r = 0; r < NV; ++r)
+            for (int q2 = 0; q2 < NV; ++q2) {
+              double sum = 0.0;
+              for (int k = 0; k < NV; ++k) sum += Lb[r * NV + k] * ilu[(size_t)(t * B + k * NV + q2) * plane + cnb];
+              ilu[(size_t)(u * B + r * NV + q2) * plane + c] -= sum;
+            }
+        }
+      }
+      double D[B], Dinv[B];
+      for (int e = 0; e < B; ++e) D[e] = ilu[(size_t)e * plane + c];
+      inv_small<NV>(D, Dinv);
+      for (int e = 0; e < B; ++e) ilu[(size_t)e * plane + c] = Dinv[e];
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// mode 0: z = (LU)^-1 b ;  mode 1: z = (LU)^-T b  (transposed, for the adjoint)
+template <int NV>
+__global__ __launch_bounds__(1024) void lilu_solve_kernel(int nic, int njc, int ldi, int mode, const double* ilu,
+                                                          const double* b, double* z) {
+  constexpr int B = NV * NV;
+  const size_t plane = (size_t)njc * ldi;
+  const int nlev = nic + 2 * (njc - 1);
+  for (int pass = 0; pass < 2; ++pass) {
+    // mode 0: pass 0 = L (ascending), pass 1 = U (descending)
+    // mode 1: pass 0 = U^T (ascending), pass 1 = L^T (descending)
+    const bool asc = pass == 0;
+    for (int step = 0; step < nlev; ++step) {
+      const int L = asc ? step : nlev - 1 - step;
+      const int jlo = max(0, (L - nic + 2) / 2), jhi = min(njc - 1, L / 2);
+      for (int j = jlo + (int)threadIdx.x; j <= jhi; j += blockDim.x) {
+        const int i = L - 2 * j;
+        if (i < 0 || i >= nic) continue;
+        const size_t c = (size_t)j * ldi + i;
+        double acc[NV];
+        for (int k = 0; k < NV; ++k) acc[k] = (pass == 0) ? b[k * plane + c] : z[k * plane + c];
+        for (int s = 1; s < NSLOT; ++s) {
+          const int di = kSlotDi[s], dj = kSlotDj[s];
+          const bool low = is_lower(di, dj);
+          if (low != asc) continue;  // ascending passes use earlier-level neighbours
+          const int ni_ = i + di, nj_ = j + dj;
+          if (ni_ < 0 || ni_ >= nic || nj_ < 0 || nj_ >= njc) continue;
+          const size_t cnb = (size_t)nj_ * ldi + ni_;
+          double zn[NV];
+          for (int m = 0; m < NV; ++m) zn[m] = z[m * plane + cnb];
+          if (mode == 0) {
+            for (int k = 0; k < NV; ++k) {
+              double sum = 0.0;
+              for (int m = 0; m < NV; ++m) sum += ilu[(size_t)(s * B + k * NV + m) * plane + c] * zn[m];
+              acc[k] -= sum;
+            }
+          } else {
+            // transposed: the entry of row n at the slot pointing back to r
+            const int sb = slot_index(-di, -dj);
+            for (int k = 0; k < NV; ++k) {
+              double sum = 0.0;
+              for (int m = 0; m < NV; ++m) sum += ilu[(size_t)(sb * B + m * NV + k) * plane + cnb] * zn[m];
+              acc[k] -= sum;
+            }
+          }
+        }
+        const bool diag = (mode == 0) ? (pass == 1) : (pass == 0);
+        if (diag) {
+          for (int k = 0; k < NV; ++k) {
+            double sum = 0.0;
+            for (int m = 0; m < NV; ++m)
+              sum += ((mode == 0) ? ilu[(size_t)(k * NV + m) * plane + c] : ilu[(size_t)(m * NV + k) * plane + c]) * acc[m];
+            z[k * plane + c] = sum;
+          }
+        } else {
+          for (int k = 0; k < NV; ++k) z[k * plane + c] = acc[k];
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+    }
+  }
+}
+
+hipError_t lilu_factor(int nic, int njc, int ldi, int nv, const double* idt, const double* blocks, double* ilu,
+                       hipStream_t s) {
+  dim3 gi((nic + 127) / 128, njc), bi(128);
+  if (nv == 5) hipLaunchKernelGGL(ilu_init_kernel<5>, gi, bi, 0, s, nic, njc, ldi, idt, blocks, ilu);
+  else hipLaunchKernelGGL(ilu_init_kernel<4>, gi, bi, 0, s, nic, njc, ldi, idt, blocks, ilu);
+  if (nv == 5) hipLaunchKernelGGL(lilu_factor_kernel<5>, dim3(1), dim3(1024), 0, s, nic, njc, ldi, ilu);
+  else hipLaunchKernelGGL(lilu_factor_kernel<4>, dim3(1), dim3(1024), 0, s, nic, njc, ldi, ilu);
+  return hipGetLastError();
+}
+
+hipError_t lilu_apply(int nic, int njc, int ldi, int nv, const double* ilu, const double* bvec, double* z, bool transpose,
+                      hipStream_t s) {
+  if (nv == 5) hipLaunchKernelGGL(lilu_solve_kernel<5>, dim3(1), dim3(1024), 0, s, nic, njc, ldi, transpose ? 1 : 0, ilu, bvec, z);
+  else hipLaunchKernelGGL(lilu_solve_kernel<4>, dim3(1), dim3(1024), 0, s, nic, njc, ldi, transpose ? 1 : 0, ilu, bvec, z);
+  return hipGetLastError();
+}
+
+// ---- adjoint products from the Jacobian blocks ----------------------------
+// (J^T psi)_c = sum_s J[c - d_s, c]^T psi_{c - d_s}, with J[r, r + d_s]
+// stored in slot s of row r: a 13-neighbour gather, deterministic.
+template <int NV>
+__global__ void jt_gather_kernel(int nic, int njc, int ldi, const double* blocks, const double* psi, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nic) return;
+  constexpr int B = NV * NV;
+  const size_t plane = (size_t)njc * ldi, c = (size_t)j * ldi + i;
+  double acc[NV];
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int s = 0; s < NSLOT; ++s) {
+    const int ri = i - kSlotDi[s], rj = j - kSlotDj[s];
+    if (ri < 0 || ri >= nic || rj < 0 || rj >= njc) continue;
+    const size_t rc = (size_t)rj * ldi + ri;
+    double pr[NV];
+    for (int m = 0; m < NV; ++m) pr[m] = psi[m * plane + rc];
+    for (int k = 0; k < NV; ++k) {
+      double sum = 0.0;
+      for (int m = 0; m < NV; ++m) sum += blocks[(size_t)(s * B + m * NV + k) * plane + rc] * pr[m];
+      acc[k] += sum;
+    }
+  }
+  for (int k = 0; k < NV; ++k) out[k * plane + c] = acc[k];
+}
+
+hipError_t jt_gather(int nic, int njc, int ldi, int nv, const double* blocks, const double* psi, double* out,
+                     hipStream_t s) {
+  dim3 g((nic + 127) / 128, njc), b(128);
+  if (nv == 5) hipLaunchKernelGGL(jt_gather_kernel<5>, g, b, 0, s, nic, njc, ldi, blocks, psi, out);
+  else hipLaunchKernelGGL(jt_gather_kernel<4>, g, b, 0, s, nic, njc, ldi, blocks, psi, out);
+  return hipGetLastError();
+}
+
+// w = z * idt - J^T z   (transposed operator I/dt - J^T)
+// dbeta_c = psi_4c * dR4_c/dbeta_c  (dR/dbeta is diagonal: beta_c enters only
+// the SA production of cell c)
+__global__ void dbeta_kernel(int nic, int njc, int ldi, const double* psi, const double* diag, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nic) return;
+  const size_t plane = (size_t)njc * ldi, c = (size_t)j * ldi + i;
+  out[c] = psi[4 * plane + c] * diag[4 * plane + c];
+}
+hipError_t dbeta_apply(int nic, int njc, int ldi, const double* psi, const double* diag, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(dbeta_kernel, dim3((nic + 127) / 128, njc), dim3(128), 0, s, nic, njc, ldi, psi, diag, out);
+  return hipGetLastError();
+}
+
+// Transposed five-colour ILU(0): (LU)^T z = b, U^T ascending colours with the
+// transposed diagonal inverse, then L^T descending colours.
+template <int NV>
+__global__ void ilu_solve_t_kernel(int nic, int njc, int ldi, int colour, int pass, const double* ilu, const double* b,
+                                   double* z) {
+  const int j = blockIdx.y;
+  const int i = ((colour - 3 * j) % 5 + 5) % 5 + 5 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= nic) return;
+  constexpr int B = NV * NV;
+  const size_t plane = (size_t)njc * ldi, c = (size_t)j * ldi + i;
+  double acc[NV];
+  for (int k = 0; k < NV; ++k) acc[k] = pass == 0 ? b[k * plane + c] : z[k * plane + c];
+  for (int s = 1; s < NSLOT; ++s) {
+    const int ni_ = i + kSlotDi[s], nj_ = j + kSlotDj[s];
+    if (ni_ < 0 || ni_ >= nic || nj_ < 0 || nj_ >= njc) continue;
+    const int cn = colour5(ni_, nj_);
+    if (pass == 0 ? (cn >= colour) : (cn <= colour)) continue;
+    const int sb = slot_index(-kSlotDi[s], -kSlotDj[s]);  // entry of row n pointing back to this cell
+    const size_t cnb = (size_t)nj_ * ldi + ni_;
+    double zn[NV];
+    for (int m = 0; m < NV; ++m) zn[m] = z[m * plane + cnb];
+    for (int k = 0; k < NV; ++k) {
+      double sum = 0.0;
+      for (int m = 0; m < NV; ++m) sum += ilu[(size_t)(sb * B + m * NV + k) * plane + cnb] * zn[m];
+      acc[k] -= sum;
+    }
+  }
+  if (pass == 0) {
+    for (int k = 0; k < NV; ++k) {
+      double sum = 0.0;
+      for (int m = 0; m < NV; ++m) sum += ilu[(size_t)(m * NV + k) * plane + c] * acc[m];
+      z[k * plane + c] = sum;
+    }
+  } else {
+    for (int k = 0; k < NV; ++k) z[k * plane + c] = acc[k];
+  }
+}
+
+hipError_t ilu_apply_t(int nic, int njc, int ldi, int nv, const double* ilu, const double* bvec, double* z, hipStream_t s) {
+  const int per_row = (nic + 4) / 5;
+  dim3 g((per_row + 63) / 64, njc), b(64);
+  for (int c = 0; c < 5; ++c) {
+    if (nv == 5) hipLaunchKernelGGL(ilu_solve_t_kernel<5>, g, b, 0, s, nic, njc, ldi, c, 0, ilu, bvec, z);
+    else hipLaunchKernelGGL(ilu_solve_t_kernel<4>, g, b, 0, s, nic, njc, ldi, c, 0, ilu, bvec, z);
+  }
+  for (int c = 4; c >= 0; --c) {
+    if (nv == 5) hipLaunchKernelGGL(ilu_solve_t_kernel<5>, g, b, 0, s, nic, njc, ldi, c, 1, ilu, bvec, z);
+    else hipLaunchKernelGGL(ilu_solve_t_kernel<4>, g, b, 0, s, nic, njc, ldi, c, 1, ilu, bvec, z);
+  }
+  return hipGetLastError();
+}
+
+// Largest step fraction alpha <= 1 with |d rho| <= f rho and |d p| <= f p per
+// cell (d p linearised: (gamma-1)(dE - u d(rho u) - v d(rho v) + |u|^2/2 d rho));
+// per-block minima, reduced on the host (min is order independent).
+__global__ __launch_bounds__(256) void limit_kernel(int nic, int njc, int ldi, const double* q, const double* dq,
+                                                    double f, double* block_min) {
+  __shared__ double red[256];
+  const size_t plane = (size_t)njc * ldi;
+  double a = 1.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < (size_t)nic * njc; e += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / nic), i = (int)(e % nic);
+    const size_t c = (size_t)j * ldi + i;
+    const double r = q[c], ru = q[plane + c], rv = q[2 * plane + c], rE = q[3 * plane + c];
+    const double u = ru / r, v = rv / r, p = 0.4 * (rE - 0.5 * r * (u * u + v * v));
+    const double dr = dq[c], dp = 0.4 * (dq[3 * plane + c] - u * dq[plane + c] - v * dq[2 * plane + c] +
+                                         0.5 * (u * u + v * v) * dr);
+    if (!(r > 0.0) || !(p > 0.0)) { a = 0.0; continue; }
+    if (fabs(dr) > f * r) a = fmin(a, f * r / fabs(dr));
+    if (fabs(dp) > f * p) a = fmin(a, f * p / fabs(dp));
+    if (!isfinite(dr) || !isfinite(dp)) a = 0.0;
+  }
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) block_min[blockIdx.x] = red[0];
+}
+hipError_t update_limit(int nic, int njc, int ldi, const double* q, const double* dq, double f, double* block_min,
+                        int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(limit_kernel, dim3(nblocks), dim3(256), 0, s, nic, njc, ldi, q, dq, f, block_min);
+  return hipGetLastError();
+}
+
 // inverse local time step per cell: idt = 1/dt (dt from the dt kernel, SoA plane 0)
 __global__ void invert_kernel(const double* dt, double* idt, int nic, int njc, int ldi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;

@@ -34,6 +34,20 @@ hipError_t line_gs(int nic, int njc, int ldi, int nv, int colour, const double* 
 hipError_t ilu_factor(int nic, int njc, int ldi, int nv, const double* idt, const double* blocks, double* ilu,
                       hipStream_t s);
 hipError_t ilu_apply(int nic, int njc, int ldi, int nv, const double* ilu, const double* b, double* z, hipStream_t s);
+// level-scheduled block ILU(0) of I/dt - J in natural j-major order (one
+// persistent work-group); apply with transpose = true solves with (LU)^T
+hipError_t lilu_factor(int nic, int njc, int ldi, int nv, const double* idt, const double* blocks, double* ilu,
+                       hipStream_t s);
+hipError_t lilu_apply(int nic, int njc, int ldi, int nv, const double* ilu, const double* b, double* z, bool transpose,
+                      hipStream_t s);
+// adjoint helpers
+hipError_t jt_gather(int nic, int njc, int ldi, int nv, const double* blocks, const double* psi, double* out,
+                     hipStream_t s);
+hipError_t dbeta_apply(int nic, int njc, int ldi, const double* psi, const double* diag, double* out, hipStream_t s);
+hipError_t ilu_apply_t(int nic, int njc, int ldi, int nv, const double* ilu, const double* b, double* z, hipStream_t s);
+// per-block minima of the admissible update fraction (see sg_linalg.hip)
+hipError_t update_limit(int nic, int njc, int ldi, const double* q, const double* dq, double f, double* block_min,
+                        int nblocks, hipStream_t s);
 hipError_t invert_dt(const double* dt, double* idt, int nic, int njc, int ldi, hipStream_t s);
 
 }  // namespace sg

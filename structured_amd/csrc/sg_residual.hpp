@@ -133,6 +133,14 @@ __device__ __forceinline__ int ghost_index(int ci, int cj, int nic, int njc) {
   return 2 * (nic + 2) + njc + gj - 1;
 }
 
+// A pure tangent (value 0) for the dual, nothing for plain doubles.
+template <class S>
+__device__ __forceinline__ S beta_tangent(double t);
+template <>
+__device__ __forceinline__ double beta_tangent<double>(double) { return 0.0; }
+template <>
+__device__ __forceinline__ Dual beta_tangent<Dual>(double t) { return Dual(0.0, t); }
+
 // LDS slots of the staged cell data.
 enum { L_R = 0, L_U, L_V, L_P, L_T, L_MU, L_K, L_N, L_ETA, L_ML, L_MAX };
 
@@ -246,7 +254,7 @@ __device__ __forceinline__ void transport(const Prim<S>& P, const DevPhys& ph, S
 // gradients are Green-Gauss over its four faces with arithmetic face means.
 template <class S>
 __device__ __forceinline__ S sa_source(S r, S n, S ml, const S* gu, const S* gv, const S* gn, const S* gr, double d,
-                                       double b) {
+                                       S b) {
   const double cw1 = kCb1 / (kKappa * kKappa) + (1.0 + kCb2) / kSigma;
   S nu = sdiv(ml, r);
   S omega = sabs(gv[0] - gu[1]);
@@ -279,13 +287,13 @@ __device__ __forceinline__ S sa_source(S r, S n, S ml, const S* gu, const S* gv,
 
 template <class S, bool SA>
 __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, DevBc bc, Field<S> q, OutField<S> out,
-                                                       double* partial) {
+                                                       double* partial, const double* beta_dot) {
   constexpr int NL = SA ? L_MAX : (L_K + 1);
   constexpr int NV = SA ? 5 : 4;
   constexpr int NLDS = NL * NPOS > 2 * NV * NT ? NL * NPOS : 2 * NV * NT;
   __shared__ double lds_raw[NLDS * (sizeof(S) / sizeof(double))];
   S* lds = reinterpret_cast<S*>(lds_raw);
-  __shared__ double red[NT / 64][NV];
+  __shared__ double red[NT / 64][2 * NV];
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
@@ -472,7 +480,9 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
     S gu[2], gv[2], gn[2], gr[2];
     cgrad(L_U, gu); cgrad(L_V, gv); cgrad(L_N, gn); cgrad(L_R, gr);
     const size_t cidx = (size_t)cj * g.ldi + ci;
-    src = sa_source<S>(at(L_R, x, y), at(L_N, x, y), at(L_ML, x, y), gu, gv, gn, gr, g.wd[cidx], g.beta[cidx]);
+    S bcell(g.beta[cidx]);
+    if (beta_dot) bcell = bcell + beta_tangent<S>(beta_dot[cidx]);
+    src = sa_source<S>(at(L_R, x, y), at(L_N, x, y), at(L_ML, x, y), gu, gv, gn, gr, g.wd[cidx], bcell);
   }
 
   // ---- exchange face fluxes through LDS (reusing the staging area)
@@ -484,9 +494,9 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
   }
   __syncthreads();
 
-  double sq[NV];
+  double sq[2 * NV];  // R^2 (reference norm) and (V R)^2 (flux-balance norm)
 #pragma unroll
-  for (int k = 0; k < NV; ++k) sq[k] = 0.0;
+  for (int k = 0; k < 2 * NV; ++k) sq[k] = 0.0;
   if (owns) {
     const double V = vol(ci, cj), iV = frcp(V);
     const size_t cidx = (size_t)cj * g.ldi + ci;
@@ -502,22 +512,23 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
       r = r * iV;
       out.put(k * g.plane + cidx, r);
       sq[k] = val(r) * val(r);
+      sq[NV + k] = sq[k] * V * V;
     }
   }
   // ---- block partial sums of R^2 (fixed order: wavefront butterfly, then waves)
   if (partial) {
     const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
+    for (int k = 0; k < 2 * NV; ++k) {
       double s = sq[k];
       for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
       if (lane == 0) red[wv][k] = s;
     }
     __syncthreads();
-    if (tid < NV) {
+    if (tid < 2 * NV) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
-      partial[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * NV + tid] = s;
+      partial[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * 2 * NV + tid] = s;
     }
   }
 }

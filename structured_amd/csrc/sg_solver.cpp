@@ -40,10 +40,13 @@ using namespace sg;
 struct SolverState {
   int m = 0;                 // restart length
   size_t n = 0;              // vector length (nvar * plane)
-  DevBuf V, Z, blocks, fac, ilu, idt, dtv, rhs, x, tmp, jz, qop, seed, hdev, ydev, partial, gst;
+  DevBuf V, Z, blocks, fac, ilu, idt, dtv, rhs, x, tmp, jz, qop, seed, hdev, ydev, partial, gst, dbdiag, ones;
   bool have_lhs = false, have_rhs = false;
-  int precond = 4;           // 0 none, 1 block Jacobi, 2 j-line, 3 line Gauss-Seidel, 4 multicolour ILU(0)
+  int precond = 5;           // 0 none, 1 block Jacobi, 2 j-line, 3 line Gauss-Seidel, 4 multicolour ILU(0),
+                             // 5 level-scheduled natural-order ILU(0)
   int sweeps = 1;
+  double pc_boost = 1.0;     // preconditioner uses idt * pc_boost
+  bool transpose = false;    // adjoint solves: operator and preconditioner transposed
   double rtol = 1e-3;
   int maxit = 60;
   int last_iters = 0;
@@ -69,7 +72,7 @@ int ensure(sg_ctx* ctx, int restart) {
   S->m = restart;
   const size_t nb = ctx->g.plane * NSLOT * nv * nv, nf = ctx->g.plane * 3 * nv * nv;
   if (S->V.alloc((size_t)(restart + 1) * n) || S->Z.alloc((size_t)restart * n) || S->blocks.alloc(nb) ||
-      S->fac.alloc(nf) || S->ilu.alloc(nb) || S->gst.alloc(n) || S->idt.alloc(ctx->g.plane) || S->dtv.alloc(n) || S->rhs.alloc(n) || S->x.alloc(n) ||
+      S->fac.alloc(nf) || S->ilu.alloc(nb) || S->gst.alloc(n) || S->dbdiag.alloc(n) || S->ones.alloc(ctx->g.plane) || S->idt.alloc(ctx->g.plane) || S->dtv.alloc(n) || S->rhs.alloc(n) || S->x.alloc(n) ||
       S->tmp.alloc(n) || S->jz.alloc(n) || S->qop.alloc(n) || S->seed.alloc(n) || S->hdev.alloc(restart + 2) ||
       S->ydev.alloc(restart + 2) || S->partial.alloc(multi_dot_partial_size(restart + 2)))
     return sg_fail(ctx, SG_ERR_HIP, "solver allocation failed");
@@ -96,12 +99,22 @@ int precond_apply(sg_ctx* ctx, const double* v, double* z) {
     return SG_OK;
   }
   const int nic = ctx->geo.nic, njc = ctx->geo.njc, ldi = ctx->g.ldi, nv = ctx->nvar;
-  if (S->precond <= 2) {
+  if (S->precond == 5) {
+    SG_TRY(lilu_apply(nic, njc, ldi, nv, S->ilu.p, v, z, S->transpose, ctx->stream), "ilu apply");
+    return SG_OK;
+  }
+  if (S->precond <= 2 && !S->transpose) {
     SG_TRY(line_solve(nic, njc, ldi, nv, S->fac.p, v, z, ctx->stream), "precond");
     return SG_OK;
   }
   if (S->precond == 4) {
-    SG_TRY(ilu_apply(nic, njc, ldi, nv, S->ilu.p, v, z, ctx->stream), "ilu apply");
+    if (S->transpose) SG_TRY(ilu_apply_t(nic, njc, ldi, nv, S->ilu.p, v, z, ctx->stream), "ilu apply");
+    else SG_TRY(ilu_apply(nic, njc, ldi, nv, S->ilu.p, v, z, ctx->stream), "ilu apply");
+    return SG_OK;
+  }
+  if (S->transpose) return sg_fail(ctx, SG_ERR_ARG, "transposed solves support precond 0, 4 and 5 only");
+  if (S->precond == 5) {
+    SG_TRY(lilu_apply(nic, njc, ldi, nv, S->ilu.p, v, z, S->transpose, ctx->stream), "ilu apply");
     return SG_OK;
   }
   // symmetric multicolour line Gauss-Seidel from z = 0
@@ -116,14 +129,16 @@ int precond_apply(sg_ctx* ctx, const double* v, double* z) {
 // w = A z = z/dt - J(qop) z
 int apply_operator(sg_ctx* ctx, const double* z, double* w) {
   SolverState* S = ctx->solver;
-  SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, S->qop.p, z, nullptr, S->jz.p, ctx->stream), "jv");
+  if (S->transpose)
+    SG_TRY(jt_gather(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->nvar, S->blocks.p, z, S->jz.p, ctx->stream), "jt");
+  else
+    SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, S->qop.p, z, nullptr, S->jz.p, ctx->stream), "jv");
   SG_TRY(shift_minus(z, S->idt.p, S->jz.p, w, ctx->g.plane, ctx->nvar, ctx->stream), "shift");
   return SG_OK;
 }
 
-int build_preconditioner(sg_ctx* ctx) {
+int extract_jacobian(sg_ctx* ctx) {
   SolverState* S = ctx->solver;
-  if (S->precond == 0) return SG_OK;
   const int nic = ctx->geo.nic, njc = ctx->geo.njc, ldi = ctx->g.ldi, nv = ctx->nvar;
   for (int a = 0; a < 5; ++a)
     for (int b = 0; b < 5; ++b)
@@ -132,12 +147,36 @@ int build_preconditioner(sg_ctx* ctx) {
         SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, S->qop.p, S->seed.p, nullptr, S->jz.p, ctx->stream), "jv");
         SG_TRY(extract_blocks(nic, njc, ldi, nv, a, b, m, S->jz.p, S->blocks.p, ctx->stream), "extract");
       }
+  return SG_OK;
+}
+
+int factor_preconditioner(sg_ctx* ctx, const double* idt_pc) {
+  SolverState* S = ctx->solver;
+  if (S->precond == 0) return SG_OK;
+  const int nic = ctx->geo.nic, njc = ctx->geo.njc, ldi = ctx->g.ldi, nv = ctx->nvar;
+  const double* pc_idt = idt_pc;
+  if (S->pc_boost > 1.0) {  // boosted diagonal for the factorisation only
+    SG_TRY(axpby(S->pc_boost, idt_pc, 0.0, S->gst.p, ctx->g.plane, ctx->stream), "boost");
+    pc_idt = S->gst.p;
+  }
   if (S->precond == 4) {
-    SG_TRY(ilu_factor(nic, njc, ldi, nv, S->idt.p, S->blocks.p, S->ilu.p, ctx->stream), "ilu factor");
+    SG_TRY(ilu_factor(nic, njc, ldi, nv, pc_idt, S->blocks.p, S->ilu.p, ctx->stream), "ilu factor");
     return SG_OK;
   }
-  SG_TRY(line_factor(nic, njc, ldi, nv, S->idt.p, S->blocks.p, S->fac.p, S->precond == 1, ctx->stream), "factor");
+  if (S->precond == 5) {
+    SG_TRY(lilu_factor(nic, njc, ldi, nv, pc_idt, S->blocks.p, S->ilu.p, ctx->stream), "ilu factor");
+    return SG_OK;
+  }
+  SG_TRY(line_factor(nic, njc, ldi, nv, pc_idt, S->blocks.p, S->fac.p, S->precond == 1, ctx->stream), "factor");
   return SG_OK;
+}
+
+int build_preconditioner(sg_ctx* ctx) {
+  SolverState* S = ctx->solver;
+  if (S->precond == 0) return SG_OK;
+  int rc;
+  if ((rc = extract_jacobian(ctx))) return rc;
+  return factor_preconditioner(ctx, S->idt.p);
 }
 
 // Right-preconditioned restarted FGMRES with CGS2.  Solves A x = rhs.
@@ -253,10 +292,11 @@ extern "C" {
 
 // ---- LinearSolverGMRES: the reference's linear-solver interface ----------
 int sg_ls_preallocate(sg_ctx* ctx, int restart, int precond, int maxit, double rtol) {
-  if (!ctx || restart < 1 || restart > 200 || precond < 0 || precond > 4) return SG_ERR_ARG;
+  if (!ctx || restart < 1 || restart > 200 || precond < 0 || precond > 5) return SG_ERR_ARG;
   int rc = ensure(ctx, restart);
   if (rc) return rc;
   ctx->solver->precond = precond;
+  ctx->solver->pc_boost = 1.0;
   ctx->solver->maxit = maxit > 0 ? maxit : restart;
   ctx->solver->rtol = rtol > 0 ? rtol : 1e-3;
   return SG_OK;
@@ -329,9 +369,10 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
   S->l2_hist.clear();
   S->cfl_hist.clear();
   sg_solve_report R{};
-  double l2[8], l2_0[8];
+  double l2[16], l2_0[16];
   int lin_total = 0;
-  double ser_ref = 0.0;
+  double ser_ref = 0.0, cfl_factor = 1.0, last_alpha = 1.0;
+  int recoveries = 0;
   bool converged = false;
   int it = 0;
   for (;; ++it) {
@@ -340,7 +381,7 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     SG_TRY(invert_dt(S->dtv.p, S->idt.p, ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->stream), "invert");
     SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q, S->rhs.p, ctx->partial.p, nullptr, ctx->l2.p,
                            ctx->stream), "residual");
-    SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
     SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
     t_res += tr.s();
     if (it == 0) std::copy(l2, l2 + nv, l2_0);
@@ -348,14 +389,25 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     S->cfl_hist.push_back(cfl);
     bool finite = true;
     double lmax = 0.0;
-    for (int k = 0; k < nv; ++k) { finite = finite && std::isfinite(l2[k]); lmax = std::max(lmax, l2[k]); }
+    for (int k = 0; k < nv; ++k) {
+      finite = finite && std::isfinite(l2[k]);
+      lmax = std::max(lmax, o->norm_weighted ? l2[nv + k] : l2[k]);
+    }
     if (o->verbose)
-      std::fprintf(stderr, "newton %4d  cfl %.3e  |R| %.3e %.3e %.3e %.3e %.3e  lin %d (%.1e)\n", it, cfl, l2[0], l2[1],
-                   l2[2], l2[3], nv > 4 ? l2[4] : 0.0, S->last_iters, S->last_relres);
-    if (!finite) { rc = sg_fail(ctx, SG_ERR_NAN, "non-finite residual in the Newton loop"); break; }
+      std::fprintf(stderr, "newton %4d  cfl %.3e  |R| %.3e %.3e %.3e %.3e %.3e  lin %d (%.1e) alpha %.2f\n", it, cfl,
+                   l2[0], l2[1], l2[2], l2[3], nv > 4 ? l2[4] : 0.0, S->last_iters, S->last_relres, last_alpha);
+    if (!finite) {
+      // recovery: return to the last linearisation state and cut the CFL
+      if (it == 0 || ++recoveries > 12) { rc = sg_fail(ctx, SG_ERR_NAN, "non-finite residual in the Newton loop"); break; }
+      SG_TRY(hipMemcpyAsync(q, S->qop.p, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "restore");
+      cfl_factor *= 0.1;
+      cfl = std::max(o->cfl, cfl * 0.1);
+      continue;
+    }
     if (lmax < o->tolerance) { converged = true; break; }
     if (it >= o->iteration_max) break;
     Timer tj;
+    S->pc_boost = (o->pc_cfl_max > 0.0 && cfl > o->pc_cfl_max) ? cfl / o->pc_cfl_max : 1.0;
     SG_TRY(hipMemcpyAsync(S->qop.p, q, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
     if ((rc = build_preconditioner(ctx))) return rc;
     SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
@@ -367,7 +419,21 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     S->last_iters = li;
     S->last_relres = rr;
     lin_total += li;
-    SG_TRY(axpby(ur, S->x.p, 1.0, q, n, ctx->stream), "update");
+    // update limiter: keep rho and p changes within 20% per cell
+    double alpha = 1.0;
+    {
+      const int nbk = 256;
+      SG_TRY(update_limit(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, q, S->x.p, 0.2 / std::max(ur, 1e-12), S->hdev.p,
+                          std::min(nbk, S->m + 2), ctx->stream), "limit");
+      std::vector<double> bm(std::min(nbk, S->m + 2));
+      SG_TRY(hipMemcpyAsync(bm.data(), S->hdev.p, bm.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+      SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+      for (double b : bm) alpha = std::min(alpha, b);
+    }
+    if (alpha < 1.0) cfl_factor = std::max(1e-6, cfl_factor * 0.5);
+    else cfl_factor = std::min(1.0, cfl_factor * 2.0);
+    last_alpha = alpha;
+    SG_TRY(axpby(ur * alpha, S->x.p, 1.0, q, n, ctx->stream), "update");
     t_lin += tl.s();
     // ramps, src/solver/solver.cpp:212-230 (counter = it + 1 after the step)
     const int counter = it + 1;
@@ -378,7 +444,7 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
       tot = std::sqrt(tot);
       ser_ref = std::max(ser_ref, tot);
       cfl = std::min(o->cfl * std::pow(ser_ref / std::max(tot, 1e-300), o->ser_exponent), o->cfl_max);
-      cfl = std::max(cfl, o->cfl);
+      cfl = std::max(cfl * cfl_factor, o->cfl);
     }
     if (o->under_relaxation_ramp && counter > o->under_relaxation_ramp_iteration)
       ur = std::min(std::pow(ur, o->under_relaxation_ramp_exponent), 10.0);
@@ -387,7 +453,11 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
   R.iterations = it;
   R.converged = converged ? 1 : 0;
   R.linear_iterations = lin_total;
-  for (int k = 0; k < nv && k < 5; ++k) { R.l2_final[k] = l2[k]; R.l2_initial[k] = l2_0[k]; }
+  for (int k = 0; k < nv && k < 5; ++k) {
+    R.l2_final[k] = l2[k];
+    R.l2_initial[k] = l2_0[k];
+    R.l2w_final[k] = l2[nv + k];
+  }
   R.wall_s = wall.s();
   R.residual_s = t_res;
   R.jacobian_s = t_jac;
@@ -407,6 +477,183 @@ int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos) {
   if ((rc = upload(ctx, v_aos, S->tmp.p))) return rc;
   if ((rc = precond_apply(ctx, S->tmp.p, S->x.p))) return rc;
   return download(ctx, S->x.p, z_aos);
+}
+
+
+// ---- adjoint -------------------------------------------------------------
+namespace {
+// Blocks of J and the diagonal of dR/dbeta at q (device SoA): the state of
+// every following J^T product.
+int adjoint_prepare(sg_ctx* ctx, const double* q_dev) {
+  if (!ctx->bcc.local)
+    return sg_fail(ctx, SG_ERR_ARG, "adjoint products need boundary conditions whose ghost cells depend on nearby cells "
+                                    "only (no wake or wide periodic boundaries)");
+  int rc;
+  if (!ctx->solver && (rc = ensure(ctx, 30))) return rc;
+  SolverState* S = ctx->solver;
+  SG_TRY(hipMemcpyAsync(S->qop.p, q_dev, S->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+  if ((rc = extract_jacobian(ctx))) return rc;
+  if (ctx->sa) {
+    SG_TRY(hipMemsetAsync(S->seed.p, 0, S->n * sizeof(double), ctx->stream), "memset");
+    std::vector<double> one(ctx->g.plane, 1.0);
+    SG_TRY(hipMemcpyAsync(S->ones.p, one.data(), one.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "H2D");
+    SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, true, S->qop.p, S->seed.p, nullptr, S->dbdiag.p, ctx->stream, S->ones.p),
+           "dbeta");
+    SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  }
+  return SG_OK;
+}
+}  // namespace
+
+int sg_jtv_dev(sg_ctx* ctx, const double* q_dev, const double* psi_dev, double* jtpsi_dev, double* dbeta_dev) {
+  if (!ctx || !q_dev || !psi_dev || !jtpsi_dev) return SG_ERR_ARG;
+  int rc;
+  if ((rc = adjoint_prepare(ctx, q_dev))) return rc;
+  SG_TRY(jt_gather(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->nvar, ctx->solver->blocks.p, psi_dev, jtpsi_dev,
+                   ctx->stream), "jt");
+  if (dbeta_dev && ctx->sa)
+    SG_TRY(dbeta_apply(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, psi_dev, ctx->solver->dbdiag.p, dbeta_dev, ctx->stream),
+           "dbeta");
+  SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  return SG_OK;
+}
+
+int sg_jtv(sg_ctx* ctx, const double* q_aos, const double* psi_aos, double* jtpsi_aos, double* dbeta) {
+  if (!ctx || !q_aos || !psi_aos || !jtpsi_aos) return SG_ERR_ARG;
+  int rc;
+  if (!ctx->solver && (rc = ensure(ctx, 30))) return rc;
+  SolverState* S = ctx->solver;
+  if ((rc = upload(ctx, q_aos, ctx->q.p))) return rc;
+  if ((rc = upload(ctx, psi_aos, ctx->v.p))) return rc;
+  if ((rc = adjoint_prepare(ctx, ctx->q.p))) return rc;
+  SG_TRY(jt_gather(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->nvar, S->blocks.p, ctx->v.p, ctx->w.p, ctx->stream), "jt");
+  if ((rc = download(ctx, ctx->w.p, jtpsi_aos))) return rc;
+  if (dbeta) {
+    const HostGeom& G = ctx->geo;
+    if (!ctx->sa) {
+      std::fill(dbeta, dbeta + (size_t)G.nic * G.njc, 0.0);
+    } else {
+      SG_TRY(dbeta_apply(G.nic, G.njc, ctx->g.ldi, ctx->v.p, S->dbdiag.p, S->tmp.p, ctx->stream), "dbeta");
+      std::vector<double> h(ctx->g.plane);
+      SG_TRY(hipMemcpyAsync(h.data(), S->tmp.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+      SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+      for (int i = 0; i < G.nic; ++i)
+        for (int j = 0; j < G.njc; ++j) dbeta[(size_t)i * G.njc + j] = h[(size_t)j * ctx->g.ldi + i];
+    }
+  }
+  return SG_OK;
+}
+
+// Solve J^T psi = rhs at q (steady adjoint).  A = -J (1/dt = 0), the
+// preconditioner factors I/dt_pc - J at the local time step of CFL pc_cfl.
+int sg_adjoint_solve(sg_ctx* ctx, const double* q_aos, const double* rhs_aos, double* psi_aos,
+                     const sg_adjoint_opts* o, sg_adjoint_report* rep) {
+  if (!ctx || !q_aos || !rhs_aos || !psi_aos || !o) return SG_ERR_ARG;
+  if (o->precond != 0 && o->precond != 4 && o->precond != 5) return sg_fail(ctx, SG_ERR_ARG, "adjoint precond must be 0, 4 or 5");
+  int rc;
+  if ((rc = sg_ls_preallocate(ctx, o->restart, o->precond, o->maxit, o->rtol))) return rc;
+  SolverState* S = ctx->solver;
+  Timer t;
+  if ((rc = upload(ctx, q_aos, ctx->q.p))) return rc;
+  if ((rc = adjoint_prepare(ctx, ctx->q.p))) return rc;
+  // preconditioner at CFL pc_cfl
+  SG_TRY(launch_dt(ctx->g, ctx->ph.mu_inf, ctx->nvar, ctx->q.p, ctx->dsmin.p, o->pc_cfl, S->dtv.p, ctx->stream), "dt");
+  SG_TRY(invert_dt(S->dtv.p, S->idt.p, ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->stream), "invert");
+  S->pc_boost = 1.0;
+  if ((rc = factor_preconditioner(ctx, S->idt.p))) return rc;
+  SG_TRY(hipMemsetAsync(S->idt.p, 0, ctx->g.plane * sizeof(double), ctx->stream), "memset");  // steady operator
+  if ((rc = upload(ctx, rhs_aos, S->rhs.p))) return rc;
+  SG_TRY(axpby(0.0, S->rhs.p, -1.0, S->rhs.p, S->n, ctx->stream), "negate");  // (-J)^T psi = -rhs
+  S->transpose = true;
+  int it = 0;
+  double rr = 0.0;
+  rc = fgmres(ctx, o->rtol, o->maxit, &it, &rr);
+  S->transpose = false;
+  if (rc) return rc;
+  if ((rc = download(ctx, S->x.p, psi_aos))) return rc;
+  if (rep) {
+    rep->iterations = it;
+    rep->relres = rr;
+    rep->wall_s = t.s();
+  }
+  return rr <= o->rtol ? SG_OK : sg_fail(ctx, SG_ERR_NOCONV, "adjoint GMRES did not reach the tolerance");
+}
+
+// F = sum over wall cells of ds (Cf - Cf*)^2 and dF/dq (AoS), exactly: Cf of
+// wall cell i depends on cells i-1..i+1 of the first two rows only, so dCf/dq
+// comes from 3 x 2 x nvar coloured tangents of the wall kernel.
+int sg_objective_cf(sg_ctx* ctx, const double* q_aos, const double* cf_target, double* F, double* dFdq_aos) {
+  if (!ctx || !q_aos || !cf_target || !F) return SG_ERR_ARG;
+  if (!ctx->ph.viscous) return sg_fail(ctx, SG_ERR_ARG, "wall objective needs a viscous case");
+  const HostGeom& G = ctx->geo;
+  const long tail = ctx->opts.tail, first = tail - 1, nb = (long)G.ni - 2 * tail + 1;
+  if (first < 0 || nb <= 0 || first + nb > G.nic) return sg_fail(ctx, SG_ERR_ARG, "tail outside the grid");
+  int rc;
+  if ((rc = upload(ctx, q_aos, ctx->q.p))) return rc;
+  const double qinf = 0.5 * ctx->fs.rho_inf * (ctx->fs.u_inf * ctx->fs.u_inf + ctx->fs.v_inf * ctx->fs.v_inf);
+  SG_TRY(launch_wall(ctx->g, ctx->ph, ctx->bc, ctx->sa, ctx->q.p, (int)first, (int)nb, ctx->xv0.p, ctx->yv0.p, qinf,
+                     ctx->wall_cp.p, ctx->wall_cf.p, ctx->wall_f.p, ctx->stream), "wall");
+  std::vector<double> cf(nb), ds(nb), e(nb);
+  SG_TRY(hipMemcpyAsync(cf.data(), ctx->wall_cf.p, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  double f = 0.0;
+  for (long n = 0; n < nb; ++n) {
+    const size_t i = (size_t)(first + n);
+    const double* ne = &G.normal_eta[(i * G.nj + 0) * 2];
+    ds[n] = std::sqrt(ne[0] * ne[0] + ne[1] * ne[1]);
+    e[n] = cf[n] - cf_target[n];
+    f += ds[n] * e[n] * e[n];
+  }
+  *F = f;
+  if (!dFdq_aos) return SG_OK;
+  const int nv = ctx->nvar, nic = G.nic, njc = G.njc;
+  std::fill(dFdq_aos, dFdq_aos + (size_t)nic * njc * nv, 0.0);
+  std::vector<double> seed(ctx->nsoa()), dcf(nb);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 2 && b < njc; ++b)
+      for (int k = 0; k < nv; ++k) {
+        std::fill(seed.begin(), seed.end(), 0.0);
+        for (int i = a; i < nic; i += 3) seed[(size_t)k * ctx->g.plane + (size_t)b * ctx->g.ldi + i] = 1.0;
+        SG_TRY(hipMemcpyAsync(ctx->v.p, seed.data(), seed.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "H2D");
+        SG_TRY(launch_wall_tangent(ctx->g, ctx->ph, ctx->bc, ctx->sa, ctx->q.p, ctx->v.p, (int)first, (int)nb, ctx->xv0.p,
+                                   ctx->yv0.p, qinf, ctx->wall_cf.p, ctx->stream), "wall tangent");
+        SG_TRY(hipMemcpyAsync(dcf.data(), ctx->wall_cf.p, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+        for (long n = 0; n < nb; ++n) {
+          const int ci = (int)(first + n);
+          for (int c = ci - 1; c <= ci + 1; ++c) {
+            if (c < 0 || c >= nic || ((c % 3) + 3) % 3 != a) continue;
+            dFdq_aos[((size_t)c * njc + b) * nv + k] += 2.0 * ds[n] * e[n] * dcf[n];
+          }
+        }
+      }
+  return SG_OK;
+}
+
+// dF/dbeta for F = wall Cf mismatch at a converged state q: one adjoint solve
+// J^T psi = dF/dq, then dF/dbeta_c = -psi_4c dR4_c/dbeta_c.
+int sg_adjoint_gradient(sg_ctx* ctx, const double* q_aos, const double* cf_target, double* grad_beta, double* F,
+                        const sg_adjoint_opts* o, sg_adjoint_report* rep) {
+  if (!ctx || !q_aos || !cf_target || !grad_beta || !o) return SG_ERR_ARG;
+  if (!ctx->sa) return sg_fail(ctx, SG_ERR_ARG, "beta gradients need the SA model");
+  const size_t n = ctx->ncell() * ctx->nvar;
+  std::vector<double> g(n), psi(n);
+  double f = 0.0;
+  int rc = sg_objective_cf(ctx, q_aos, cf_target, &f, g.data());
+  if (rc) return rc;
+  if (F) *F = f;
+  int rs = sg_adjoint_solve(ctx, q_aos, g.data(), psi.data(), o, rep);
+  if (rs && rs != SG_ERR_NOCONV) return rs;
+  SolverState* S = ctx->solver;
+  if ((rc = upload(ctx, psi.data(), ctx->v.p))) return rc;
+  SG_TRY(dbeta_apply(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->v.p, S->dbdiag.p, S->tmp.p, ctx->stream), "dbeta");
+  std::vector<double> h(ctx->g.plane);
+  SG_TRY(hipMemcpyAsync(h.data(), S->tmp.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
+  const HostGeom& G = ctx->geo;
+  for (int i = 0; i < G.nic; ++i)
+    for (int j = 0; j < G.njc; ++j) grad_beta[(size_t)i * G.njc + j] = -h[(size_t)j * ctx->g.ldi + i];
+  return rs;
 }
 
 int sg_solve_history(sg_ctx* ctx, double* l2_hist, double* cfl_hist, int max_entries) {

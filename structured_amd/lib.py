@@ -206,6 +206,7 @@ class Equation:
         report = {f: getattr(rep, f) for f, _ in abi.SolveReport._fields_}
         report["l2_initial"] = list(rep.l2_initial)[: self.nvar]
         report["l2_final"] = list(rep.l2_final)[: self.nvar]
+        report["l2w_final"] = list(rep.l2w_final)[: self.nvar]
         return q, report, {"l2": l2h[:n], "cfl": cfl[:n]}
 
     def ls_preallocate(self, restart=30, precond=2, maxit=60, rtol=1e-3):
@@ -230,3 +231,36 @@ class Equation:
         z = np.empty(self.shape)
         self._check(load().sg_ls_precond_apply(self.h, _p(_f64(v, self.shape)), _p(z)))
         return z
+
+    # ---- adjoint
+    def adjoint_solve(self, q, rhs, restart=40, maxit=400, rtol=1e-10, precond=4, pc_cfl=1e3):
+        q = _f64(q, self.shape)
+        rhs = _f64(rhs, self.shape)
+        psi = np.empty(self.shape)
+        o = abi.AdjointOpts(restart=restart, maxit=maxit, rtol=rtol, precond=precond, reserved=0, pc_cfl=pc_cfl)
+        rep = abi.AdjointReport()
+        rc = load().sg_adjoint_solve(self.h, _p(q), _p(rhs), _p(psi), C.byref(o), C.byref(rep))
+        if rc not in (0, -5):
+            self._check(rc)
+        return psi, {"iterations": rep.iterations, "relres": rep.relres, "wall_s": rep.wall_s, "converged": rc == 0}
+
+    def objective_cf(self, q, cf_target, grad=True):
+        q = _f64(q, self.shape)
+        t = _f64(cf_target)
+        F = C.c_double()
+        g = np.empty(self.shape) if grad else None
+        self._check(load().sg_objective_cf(self.h, _p(q), _p(t), C.byref(F), _p(g)))
+        return F.value, g
+
+    def adjoint_gradient(self, q, cf_target, restart=40, maxit=400, rtol=1e-10, precond=4, pc_cfl=1e3):
+        q = _f64(q, self.shape)
+        t = _f64(cf_target)
+        gb = np.empty((self.nic, self.njc))
+        F = C.c_double()
+        o = abi.AdjointOpts(restart=restart, maxit=maxit, rtol=rtol, precond=precond, reserved=0, pc_cfl=pc_cfl)
+        rep = abi.AdjointReport()
+        rc = load().sg_adjoint_gradient(self.h, _p(q), _p(t), _p(gb), C.byref(F), C.byref(o), C.byref(rep))
+        if rc not in (0, -5):
+            self._check(rc)
+        return gb, F.value, {"iterations": rep.iterations, "relres": rep.relres, "wall_s": rep.wall_s,
+                             "converged": rc == 0}

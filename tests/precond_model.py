@@ -40,6 +40,8 @@ def apply(B, idt, v, precond, sweeps=1):
         return v.copy()
     if precond == 4:
         return ilu_mc5(B, idt, v)
+    if precond == 5:
+        return ilu_natural(B, idt, v)
     if precond in (1, 2):
         return np.stack([line_solve(B, idt, i, v[i], jacobi=(precond == 1)) for i in range(nic)])
     z = np.zeros_like(v)
@@ -122,3 +124,65 @@ def ilu_mc5(B, idt, v):
                         acc -= A[(i, j, ni, nj)] @ z[ni, nj]
                 z[i, j] = Dinv[(i, j)] @ acc
     return z
+
+
+def ilu_natural(B, idt, v, transpose=False):
+    """Block ILU(0) of I/dt - J in natural j-major order (o = j*nic + i)."""
+    v = np.asarray(v, dtype=np.float64)
+    nic, njc, nv = v.shape
+    order = [(i, j) for j in range(njc) for i in range(nic)]
+    pos = {c: n for n, c in enumerate(order)}
+    rows = []
+    for (i, j) in order:
+        d = {}
+        for s, (di, dj) in enumerate(SLOTS):
+            ii, jj = i + di, j + dj
+            if 0 <= ii < nic and 0 <= jj < njc:
+                blk = -B[i, j, s].copy()
+                if s == 0:
+                    blk += np.eye(nv) * idt[i, j]
+                d[pos[(ii, jj)]] = blk
+        rows.append(d)
+    N = len(order)
+    for rr in range(N):
+        d = rows[rr]
+        for k in sorted(c for c in d if c < rr):
+            d[k] = d[k] @ np.linalg.inv(rows[k][k])
+            for jc, ukj in rows[k].items():
+                if jc > k and jc in d:
+                    d[jc] = d[jc] - d[k] @ ukj
+    vv = np.array([v[c] for c in order])
+    if not transpose:
+        y = np.zeros_like(vv)
+        for rr in range(N):
+            s = vv[rr].copy()
+            for k, lk in rows[rr].items():
+                if k < rr:
+                    s -= lk @ y[k]
+            y[rr] = s
+        x = np.zeros_like(vv)
+        for rr in reversed(range(N)):
+            s = y[rr].copy()
+            for k, uk in rows[rr].items():
+                if k > rr:
+                    s -= uk @ x[k]
+            x[rr] = np.linalg.solve(rows[rr][rr], s)
+    else:
+        w = np.zeros_like(vv)
+        for rr in range(N):
+            s = vv[rr].copy()
+            for k in rows[rr]:
+                if k < rr:
+                    s -= rows[k][rr].T @ w[k]
+            w[rr] = np.linalg.solve(rows[rr][rr].T, s)
+        x = np.zeros_like(vv)
+        for rr in reversed(range(N)):
+            s = w[rr].copy()
+            for k in rows[rr]:
+                if k > rr:
+                    s -= rows[k][rr].T @ x[k]
+            x[rr] = s
+    out = np.zeros_like(v)
+    for n, c in enumerate(order):
+        out[c] = x[n]
+    return out

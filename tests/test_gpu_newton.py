@@ -71,7 +71,7 @@ def test_laminar_plate_converges_and_cf_matches_oracle():
     assert np.all(np.abs(blasius - 0.664) < 0.1)
 
 
-@pytest.mark.parametrize("precond", [1, 2, 3, 4])
+@pytest.mark.parametrize("precond", [1, 2, 3, 4, 5])
 def test_preconditioner_matches_numpy_model(precond):
     """The GPU preconditioner (blocks by colouring J.v, line factorisation,
     multicolour Gauss-Seidel) equals its numpy restatement built from the
@@ -91,3 +91,23 @@ def test_preconditioner_matches_numpy_model(precond):
     err = np.abs(z - zr).max() / np.abs(zr).max()
     print("precond", precond, "model err", err)
     assert err <= 1e-10
+
+
+@pytest.mark.parametrize("precond", [4, 5])
+def test_ilu_matches_numpy_model_sa(precond):
+    """The ILU preconditioners with the five SA unknowns per cell."""
+    import precond_model as pm
+    case = cases.sa_bump(13, 9, dy1=1e-3)
+    q = case.perturbed_state(seed=6, amp=0.01)
+    eq = lib.Equation(case)
+    dt = eq.calc_dt(q, 50.0)
+    eq.ls_preallocate(restart=10, precond=precond, maxit=10, rtol=1e-6)
+    eq.ls_set_lhs(q, dt)
+    v = np.random.default_rng(3).standard_normal(q.shape)
+    z = eq.ls_precond_apply(v)
+    J = of.jacobian(case, q, wd=eq.walldist())
+    B = pm.blocks_from_dense(J, case.nic, case.njc, case.nvar)
+    zr = pm.apply(B, 1.0 / dt[..., 0], v, precond)
+    err = np.abs(z - zr).max() / np.abs(zr).max()
+    print("SA precond", precond, "model err", err)
+    assert err <= 1e-9
