@@ -208,14 +208,18 @@ typedef struct sg_newton_opts {
   /* 0: stop on the reference's norm ||R_k||_2 (R = flux balance / V); 1: on
    * ||V R_k||_2, whose roundoff floor does not grow as cells shrink. */
   int norm_weighted;
-  int reserved2;
+  /* > 0: also stop when the stopping norm, already within 1000x of the
+   * tolerance, has not fallen below half its best earlier value during the
+   * last stall_window steps (a floor set by rounding or by a
+   * piecewise-constant closure); the report then has stalled = 1. */
+  int stall_window;
 } sg_newton_opts;
 
 typedef struct sg_solve_report {
   int iterations;
   int converged;
   int linear_iterations;
-  int reserved;
+  int stalled;
   double l2_initial[5];
   double l2_final[5];
   double l2w_final[5];   /* ||V R_k||_2 at the end */
@@ -228,9 +232,85 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* opts, sg_solve_re
  * returns the number of entries recorded. */
 int sg_solve_history(sg_ctx* ctx, double* l2_hist, double* cfl_hist, int max_entries);
 
+/* ---- field inversion (no counterpart in the reference): L-BFGS on
+ *      J(beta) = F(q(beta)) + reg * sum (beta - 1)^2 with projected Armijo
+ *      backtracking; each trial re-converges the primal by sg_solve from the
+ *      last accepted state, each accepted point costs one adjoint gradient. */
+typedef struct sg_inversion_opts {
+  int iterations;         /* L-BFGS iterations                                */
+  int memory;             /* stored correction pairs                          */
+  double reg;             /* weight of the prior term sum (beta - 1)^2        */
+  double step0;           /* max |delta beta| of the first (steepest) step    */
+  double beta_min, beta_max;  /* box the iterates are projected onto          */
+  int max_backtracks;     /* step halvings (factor 4) per line search         */
+  int reserved;
+  sg_newton_opts newton;  /* primal re-solves                                 */
+  sg_adjoint_opts adjoint;
+} sg_inversion_opts;
+typedef struct sg_inversion_report {
+  int iterations, evaluations, failed_solves, reserved;
+  double J0, J, F0, F;
+  double wall_s, primal_s, adjoint_s;
+} sg_inversion_report;
+/* q_aos: converged state at the initial beta (in), state at the final beta
+ * (out); beta [nic][njc] in/out; J_hist (iterations + 1 entries, may be NULL)
+ * receives J at every accepted iterate. */
+int sg_invert(sg_ctx* ctx, double* q_aos, const double* cf_target, double* beta, const sg_inversion_opts* opts,
+              sg_inversion_report* report, double* J_hist);
+
 /* Layout conversion between the reference AoS and the library SoA. */
 int sg_aos_to_soa_dev(sg_ctx* ctx, const double* aos_dev, double* soa_dev, int nvar);
 int sg_soa_to_aos_dev(sg_ctx* ctx, const double* soa_dev, double* aos_dev, int nvar);
+
+/* ---- closures for the SA production multiplier beta (no counterpart in the
+ *      reference, whose SA source has no multiplier).  The closure is
+ *      evaluated inside every residual and J.v launch of the context, so the
+ *      Newton solve, its Jacobian blocks and J.v all see it.
+ *
+ * Local features f (five per cell, O(1)):
+ *   f0 = |W| / (|W| + |S|), f1 = log(1 + d^2 |W| / nu) / 10,
+ *   f2 = asinh(nu~ / nu) / 8, f3 = tanh(d (u . grad p) / (|u| q)),
+ *   f4 = P / (|P| + |D| + eps)
+ * (W vorticity, S strain rate, d wall distance, P and D the SA production
+ * with beta = 1 and destruction). */
+#define SG_CLOSURE_FIELD 0     /* beta from sg_set_beta (default)             */
+#define SG_CLOSURE_ANALYTIC 1  /* beta = 1 + c6 tanh(c0 f0 + ... + c4 f4 + c5) */
+#define SG_CLOSURE_MLP 2       /* beta = 1 + MLP(f), BF16 matrix cores         */
+#define SG_MLP_NPARAM 4609     /* W1[64][5] b1[64] W2[64][64] b2[64] W3[64] b3 */
+/* coef: 7 doubles for SG_CLOSURE_ANALYTIC; mlp_params: SG_MLP_NPARAM floats
+ * (host) for SG_CLOSURE_MLP; unused arguments may be NULL. */
+int sg_set_closure(sg_ctx* ctx, int kind, const double* coef, const float* mlp_params);
+/* Features [nic][njc][5] and the beta the closure gives [nic][njc] at q
+ * (either may be NULL).  With SG_CLOSURE_FIELD, beta is the field. */
+int sg_closure_eval(sg_ctx* ctx, const double* q_aos, double* features, double* beta);
+
+/* ---- closure network training (no counterpart in the reference).  A
+ *      5 -> 64 -> 64 -> 1 tanh MLP; the 64 x 64 products of the forward and
+ *      backward passes run on the matrix cores with BF16 operands and FP32
+ *      accumulation; first/last layers, loss (mean squared error), gradient
+ *      reductions and Adam (beta1 0.9, beta2 0.999, eps 1e-8) are FP32 and
+ *      deterministic. */
+typedef struct sg_mlp sg_mlp;
+/* Glorot-uniform weights (seeded), zero biases; max_rows bounds the rows of
+ * one forward/backward pass (minibatch size). */
+int sg_mlp_create(sg_mlp** net, int max_rows, unsigned long long seed);
+void sg_mlp_destroy(sg_mlp* net);
+const char* sg_mlp_last_error(const sg_mlp* net);
+int sg_mlp_set_params(sg_mlp* net, const float* params);
+int sg_mlp_get_params(sg_mlp* net, float* params);
+/* y[n] = MLP(x[n][5]) (host arrays; any n). */
+int sg_mlp_forward(sg_mlp* net, int n, const float* x, float* y);
+/* Mean squared error over x[n][5] -> t[n] (n <= max_rows) and its gradient
+ * with respect to the SG_MLP_NPARAM parameters. */
+int sg_mlp_loss_grad(sg_mlp* net, int n, const float* x, const float* t, float* loss, float* grad);
+/* epochs of Adam over n samples (host arrays, copied to the device once) in
+ * minibatches of `batch` rows drawn by a seeded shuffle per epoch (the last
+ * partial minibatch is dropped when n > batch); loss_hist[epochs] receives
+ * the mean minibatch loss of each epoch (may be NULL). */
+int sg_mlp_train(sg_mlp* net, int n, const float* x, const float* t, int epochs, int batch, float lr,
+                 unsigned long long seed, float* loss_hist);
+/* Closure of a context from the current parameters of a network. */
+int sg_set_closure_mlp(sg_ctx* ctx, const sg_mlp* net);
 
 #ifdef __cplusplus
 }

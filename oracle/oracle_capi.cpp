@@ -41,9 +41,12 @@ void make_setup(Setup& s, int ni, int nj, const double* xv, const double* yv, co
   }
 }
 
+Closure g_closure;  // set by og_set_closure, used by every following call
+
 template <class T>
 void init_res(Residual<T>& r, const Setup& s, const double* wd, const double* beta) {
   r.init(&s.geo, s.prm, s.bcs);
+  r.clo = g_closure;
   const size_t n = s.geo.nic * s.geo.njc;
   if (wd) r.wall_dist.assign(wd, wd + n);
   else { r.wall_dist.assign(n, 1.0); if (s.prm.sa) wall_distance(s.geo, s.bcs, r.wall_dist.data()); }
@@ -53,6 +56,25 @@ void init_res(Residual<T>& r, const Setup& s, const double* wd, const double* be
 }  // namespace
 
 extern "C" {
+
+// Closure of beta for the following calls: kind 0 field, 1 analytic (coef[7]),
+// 2 MLP (params: SG_MLP_NPARAM floats, W1|b1|W2|b2|W3|b3).
+int og_set_closure(int kind, const double* coef, const float* params) {
+  Closure c;
+  c.kind = kind;
+  if (kind == 1) std::copy(coef, coef + 7, c.coef);
+  if (kind == 2) {
+    c.W1.assign(params, params + 320);
+    c.b1.assign(params + 320, params + 384);
+    c.W2.resize(4096);
+    for (int k = 0; k < 4096; ++k) c.W2[k] = bf16_round((double)params[384 + k]);
+    c.b2.assign(params + 4480, params + 4544);
+    c.W3.assign(params + 4544, params + 4608);
+    c.b3 = params[4608];
+  }
+  g_closure = c;
+  return 0;
+}
 
 int og_metrics(int ni, int nj, const double* xv, const double* yv, double* nchi, double* neta, double* vol,
                double* xc, double* yc, double* ds_chi, double* ds_eta) {
@@ -79,6 +101,19 @@ int og_residual(int ni, int nj, const double* xv, const double* yv, const sg_fre
   init_res(res, s, wd, beta);
   res.evaluate(q, r);
   if (l2) l2norm(s.geo.nic * s.geo.njc, (size_t)s.prm.nvar(), r, l2);
+  return 0;
+}
+
+// Features and beta per cell at q: out[nic][njc][6].
+int og_closure_eval(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs,
+                    int nbc, const sg_opts* o, const double* wd, const double* beta, const double* q, double* out) {
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  Residual<double> res;
+  init_res(res, s, wd, beta);
+  res.aux = out;
+  std::vector<double> r(s.geo.nic * s.geo.njc * s.prm.nvar());
+  res.evaluate(q, r.data());
   return 0;
 }
 

@@ -159,6 +159,43 @@ struct SaConst {
   double cw1() const { return cb1 / (kappa * kappa) + (1.0 + cb2) / sigma; }
 };
 
+// Closure of the SA production multiplier beta (the reference has none):
+// 0 the per-cell field, 1 beta = 1 + amp tanh(a . f + a0), 2 beta = 1 +
+// MLP(f).  The MLP is evaluated in FP64 with the rounding points of the
+// device inference kept: features to FP32, hidden layer 1 (and its tangent)
+// and W2 to BF16; everything else exact.
+struct Closure {
+  int kind = 0;
+  double coef[7] = {0, 0, 0, 0, 0, 0, 0};
+  std::vector<double> W1, b1, W2, b2, W3;  // W2 already rounded to BF16
+  double b3 = 0.0;
+};
+
+// Round to the nearest bfloat16 (8-bit significand), ties to even, via FP32.
+inline double bf16_round(double x) {
+  float f = (float)x;
+  unsigned u;
+  std::memcpy(&u, &f, 4);
+  u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+  std::memcpy(&f, &u, 4);
+  return (double)f;
+}
+inline double round_bf16(double x) { return bf16_round(x); }
+template <int N> inline Dual<N> round_bf16(const Dual<N>& a) {
+  Dual<N> r(bf16_round(a.v));
+  for (int k = 0; k < N; ++k) r.d[k] = bf16_round(a.d[k]);
+  return r;
+}
+inline double round_f32(double x) { return (double)(float)x; }
+template <int N> inline Dual<N> round_f32(const Dual<N>& a) {
+  Dual<N> r((double)(float)a.v);
+  for (int k = 0; k < N; ++k) r.d[k] = (double)(float)a.d[k];
+  return r;
+}
+template <int N> inline Dual<N> log1p(const Dual<N>& a) { Dual<N> r(std::log1p(a.v)); for (int k = 0; k < N; ++k) r.d[k] = a.d[k] / (1.0 + a.v); return r; }
+template <int N> inline Dual<N> asinh(const Dual<N>& a) { Dual<N> r(std::asinh(a.v)); double s = 1.0 / std::sqrt(1.0 + a.v * a.v); for (int k = 0; k < N; ++k) r.d[k] = s * a.d[k]; return r; }
+using std::log1p; using std::asinh;
+
 struct Params {
   double rho_inf = 1.0, u_inf = 0.0, v_inf = 0.0, p_inf = 1.0 / 1.4, T_inf = 1.0 / 1.4;
   double mu_inf = 0.0, pr_inf = 0.7, aoa = 0.0;  // aoa in radians
@@ -187,6 +224,8 @@ struct Residual {
   std::vector<Bc> bcs;
   std::vector<double> wall_dist;  // [nic][njc], SA only
   std::vector<double> beta;       // [nic][njc], SA only (1 = plain SA)
+  Closure clo;                    // beta closure (kind 0: the beta field)
+  double* aux = nullptr;          // optional [nic][njc][6]: features and beta
 
   Arr2<T> rho, u, v, p, Tt, mu, kk, nt, eta, rnt;   // ghosted
   Arr2<T> ql[5], qr[5];   // chi faces [ni][njc]: rho,u,v,p,nt
@@ -527,6 +566,44 @@ struct Residual {
     for (int k = 0; k < 2; ++k)
       out[k] = (fe * nc(i + 1, j, k) - fw * nc(i, j, k) + fn * ne(i, j + 1, k) - fs * ne(i, j, k)) / V;
   }
+  // Local features of the closure (all O(1)): vorticity/strain ratio,
+  // log wall-distance Reynolds number, asinh eddy-viscosity ratio,
+  // streamwise pressure-gradient indicator, production/destruction balance.
+  void features(size_t i, size_t j, T r, T uu, T vv, T n, T nu, const T* g_u, const T* g_v, const T* g_p, double d,
+                T P1, T D, T* f) const {
+    const double U2 = prm.u_inf * prm.u_inf + prm.v_inf * prm.v_inf, U = std::sqrt(U2);
+    T om = fabs(g_v[0] - g_u[1]);
+    T s12 = g_u[1] + g_v[0];
+    T smag = sqrt(2.0 * (g_u[0] * g_u[0] + g_v[1] * g_v[1]) + s12 * s12 + 1e-12 * U2);
+    f[0] = om / (om + smag);
+    f[1] = log1p(om * (d * d) / nu) * 0.1;
+    f[2] = asinh(n / nu) * 0.125;
+    T q2 = uu * uu + vv * vv;
+    T umag = sqrt(q2 + 1e-6 * U2);
+    T qd = 0.5 * r * q2 + 0.025 * prm.rho_inf * U2;
+    f[3] = tanh(d * (uu * g_p[0] + vv * g_p[1]) / (umag * qd));
+    f[4] = P1 / (fabs(P1) + fabs(D) + 1e-8 * prm.rho_inf * U2 * U);
+    (void)i; (void)j;
+  }
+  T mlp(const T* f) const {
+    T x[5];
+    for (int c = 0; c < 5; ++c) x[c] = round_f32(f[c]);
+    T h1[64];
+    for (int k = 0; k < 64; ++k) {
+      T a(clo.b1[k]);
+      for (int c = 0; c < 5; ++c) a = a + clo.W1[k * 5 + c] * x[c];
+      h1[k] = round_bf16(tanh(a));
+    }
+    T y(clo.b3);
+    for (int jj = 0; jj < 64; ++jj) {
+      T a(0.0);
+      for (int k = 0; k < 64; ++k) a = a + clo.W2[jj * 64 + k] * h1[k];
+      y = y + clo.W3[jj] * tanh(a + clo.b2[jj]);
+    }
+    return y;
+  }
+
+  // SA-noft2 source: beta P1 - D + cb2 and density-gradient diffusion terms.
   T sa_source(size_t i, size_t j) const {
     const size_t gi = i + 1, gj = j + 1;
     T r = rho(gi, gj), n = nt(gi, gj);
@@ -535,9 +612,9 @@ struct Residual {
     T g_u[2], g_v[2], g_n[2], g_r[2];
     cell_grad(u, i, j, g_u); cell_grad(v, i, j, g_v); cell_grad(nt, i, j, g_n); cell_grad(rho, i, j, g_r);
     T omega = fabs(g_v[0] - g_u[1]);
-    const double d = wall_dist[i * g->njc + j], b = beta[i * g->njc + j];
+    const double d = wall_dist[i * g->njc + j];
     const double k2d2 = sa.kappa * sa.kappa * d * d;
-    T P, D;
+    T P1, D;
     if (value(n) >= 0.0) {
       T chi = n / nu, chi3 = chi * chi * chi;
       T fv1 = chi3 / (chi3 + sa.cv1 * sa.cv1 * sa.cv1);
@@ -552,15 +629,34 @@ struct Residual {
       T g2 = gg * gg, g6 = g2 * g2 * g2;
       const double cw3_6 = std::pow(sa.cw3, 6);
       T fw = gg * pow((1.0 + cw3_6) / (g6 + cw3_6), 1.0 / 6.0);
-      P = b * sa.cb1 * stil * r * n;
+      P1 = sa.cb1 * stil * r * n;
       D = sa.cw1() * fw * r * (n / d) * (n / d);
     } else {
-      P = b * sa.cb1 * (1.0 - 1.2) * omega * r * n;
+      P1 = sa.cb1 * (1.0 - 1.2) * omega * r * n;
       D = -sa.cw1() * r * (n / d) * (n / d);
     }
     T cb2t = (sa.cb2 / sa.sigma) * r * (g_n[0] * g_n[0] + g_n[1] * g_n[1]);
     T cross = -(1.0 / sa.sigma) * (nu + n) * (g_r[0] * g_n[0] + g_r[1] * g_n[1]);
-    return P - D + cb2t + cross;
+    T b(beta[i * g->njc + j]);
+    T f[5];
+    if (clo.kind != 0 || aux) {
+      T g_p[2];
+      cell_grad(p, i, j, g_p);
+      features(i, j, r, u(gi, gj), v(gi, gj), n, nu, g_u, g_v, g_p, d, P1, D, f);
+    }
+    if (clo.kind == 1) {
+      T a(clo.coef[5]);
+      for (int k = 0; k < 5; ++k) a = a + clo.coef[k] * f[k];
+      b = 1.0 + clo.coef[6] * tanh(a);
+    } else if (clo.kind == 2) {
+      b = 1.0 + mlp(f);
+    }
+    if (aux) {
+      double* o = aux + (i * g->njc + j) * 6;
+      for (int k = 0; k < 5; ++k) o[k] = value(f[k]);
+      o[5] = value(b);
+    }
+    return b * P1 - D + (cb2t + cross);
   }
 
   // EulerEquation::calc_residual, src/model/eulerequation.cpp:202-232.

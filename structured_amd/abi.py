@@ -50,12 +50,12 @@ class NewtonOpts(C.Structure):
                 ("under_relaxation_ramp_exponent", C.c_double), ("iteration_max", C.c_int), ("tolerance", C.c_double),
                 ("gmres_restart", C.c_int), ("gmres_maxit", C.c_int), ("gmres_rtol", C.c_double), ("precond", C.c_int),
                 ("ser", C.c_int), ("ser_exponent", C.c_double), ("verbose", C.c_int), ("pc_cfl_max", C.c_double),
-                ("norm_weighted", C.c_int), ("reserved2", C.c_int)]
+                ("norm_weighted", C.c_int), ("stall_window", C.c_int)]
 
 
 class SolveReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("linear_iterations", C.c_int),
-                ("reserved", C.c_int), ("l2_initial", C.c_double * 5), ("l2_final", C.c_double * 5),
+                ("stalled", C.c_int), ("l2_initial", C.c_double * 5), ("l2_final", C.c_double * 5),
                 ("l2w_final", C.c_double * 5), ("cfl_final", C.c_double), ("wall_s", C.c_double), ("residual_s", C.c_double),
                 ("jacobian_s", C.c_double), ("linsolve_s", C.c_double)]
 
@@ -66,7 +66,7 @@ NEWTON_DEFAULTS = dict(cfl=1.0, cfl_ramp=0, cfl_ramp_iteration=20, cfl_ramp_expo
                        under_relaxation=1.0, under_relaxation_ramp=0, under_relaxation_ramp_iteration=20,
                        under_relaxation_ramp_exponent=1.1, iteration_max=1, tolerance=1e-10, gmres_restart=30,
                        gmres_maxit=60, gmres_rtol=1e-3, precond=4, ser=0, ser_exponent=1.0, verbose=0,
-                       pc_cfl_max=0.0, norm_weighted=0, reserved2=0)
+                       pc_cfl_max=0.0, norm_weighted=0, stall_window=0)
 
 
 def newton_opts(**kw):
@@ -85,3 +85,15 @@ class AdjointOpts(C.Structure):
 
 class AdjointReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("reserved", C.c_int), ("relres", C.c_double), ("wall_s", C.c_double)]
+
+
+class InversionOpts(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("memory", C.c_int), ("reg", C.c_double), ("step0", C.c_double),
+                ("beta_min", C.c_double), ("beta_max", C.c_double), ("max_backtracks", C.c_int),
+                ("reserved", C.c_int), ("newton", NewtonOpts), ("adjoint", AdjointOpts)]
+
+
+class InversionReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("evaluations", C.c_int), ("failed_solves", C.c_int), ("reserved", C.c_int),
+                ("J0", C.c_double), ("J", C.c_double), ("F0", C.c_double), ("F", C.c_double),
+                ("wall_s", C.c_double), ("primal_s", C.c_double), ("adjoint_s", C.c_double)]

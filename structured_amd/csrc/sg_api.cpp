@@ -8,7 +8,9 @@
 #include <limits>
 
 #include "sg_context.hpp"
+#include "sg_closure.hpp"
 #include "sg_kernels.hpp"
+#include "sg_mlp.hpp"
 
 using namespace sg;
 
@@ -455,6 +457,64 @@ static int set_cell_field(sg_ctx* ctx, DevBuf& b, const double* h) {
 }
 
 int sg_set_beta(sg_ctx* ctx, const double* beta) { return set_cell_field(ctx, ctx->beta, beta); }
+
+// Closure buffers: MLP parameters as FP32 plus the BF16 copy of W2.
+int sg_closure_alloc_mlp(sg_ctx* ctx) {
+  if (!ctx->clo_params.p) SG_TRY(ctx->clo_params.alloc((MLP_NPARAM + 1) / 2), "hipMalloc");
+  if (!ctx->clo_w2.p) SG_TRY(ctx->clo_w2.alloc(MLP_H * MLP_H / 4), "hipMalloc");
+  return SG_OK;
+}
+
+int sg_set_closure(sg_ctx* ctx, int kind, const double* coef, const float* mlp_params) {
+  if (!ctx) return SG_ERR_ARG;
+  if (!ctx->sa && kind != SG_CLOSURE_FIELD) return sg_fail(ctx, SG_ERR_ARG, "a beta closure needs the SA model");
+  DevClosure c;
+  c.kind = kind;
+  if (kind == SG_CLOSURE_ANALYTIC) {
+    if (!coef) return sg_fail(ctx, SG_ERR_ARG, "analytic closure needs 7 coefficients");
+    std::copy(coef, coef + 7, c.coef);
+  } else if (kind == SG_CLOSURE_MLP) {
+    if (!mlp_params) return sg_fail(ctx, SG_ERR_ARG, "MLP closure needs its parameters");
+    int rc;
+    if ((rc = sg_closure_alloc_mlp(ctx))) return rc;
+    SG_TRY(hipMemcpyAsync(ctx->clo_params.p, mlp_params, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice,
+                          ctx->stream), "H2D");
+    SG_TRY(mlp_w2_to_bf16(reinterpret_cast<const float*>(ctx->clo_params.p),
+                          reinterpret_cast<__bf16*>(ctx->clo_w2.p), ctx->stream), "w2 bf16");
+    SG_TRY(hipStreamSynchronize(ctx->stream), "closure");
+    c.params = reinterpret_cast<const float*>(ctx->clo_params.p);
+    c.w2 = ctx->clo_w2.p;
+  } else if (kind != SG_CLOSURE_FIELD) {
+    return sg_fail(ctx, SG_ERR_ARG, "unknown closure kind");
+  }
+  ctx->clo = c;
+  return SG_OK;
+}
+
+int sg_closure_eval(sg_ctx* ctx, const double* q_aos, double* features, double* beta) {
+  if (!ctx || !q_aos) return SG_ERR_ARG;
+  if (!ctx->sa) return sg_fail(ctx, SG_ERR_ARG, "closure features need the SA model");
+  if (!ctx->ph.viscous) return sg_fail(ctx, SG_ERR_ARG, "closure features need a viscous flow");
+  int rc;
+  if (!ctx->aux.p) SG_TRY(ctx->aux.alloc((kNFeat + 1) * ctx->g.plane), "hipMalloc");
+  if ((rc = upload_aos(ctx, q_aos, ctx->q.p))) return rc;
+  DevClosure c = ctx->clo;
+  c.aux = ctx->aux.p;
+  SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, true, ctx->q.p, ctx->r.p, nullptr, nullptr, nullptr, ctx->stream, c),
+         "residual launch");
+  const size_t nc = ctx->ncell();
+  std::vector<double> f(nc * kNFeat), b(nc);
+  SG_TRY(launch_soa_to_aos(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, kNFeat, ctx->aux.p, ctx->aos_b.p, ctx->stream),
+         "soa->aos");
+  SG_TRY(hipMemcpyAsync(f.data(), ctx->aos_b.p, f.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  SG_TRY(launch_soa_to_aos(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, 1, ctx->aux.p + kNFeat * ctx->g.plane,
+                           ctx->aos_a.p, ctx->stream), "soa->aos");
+  SG_TRY(hipMemcpyAsync(b.data(), ctx->aos_a.p, b.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  SG_TRY(hipStreamSynchronize(ctx->stream), "closure eval");
+  if (features) std::copy(f.begin(), f.end(), features);
+  if (beta) std::copy(b.begin(), b.end(), beta);
+  return SG_OK;
+}
 int sg_set_walldist(sg_ctx* ctx, const double* d) { return set_cell_field(ctx, ctx->wd, d); }
 
 int sg_get_walldist(sg_ctx* ctx, double* d) {
@@ -481,7 +541,7 @@ int sg_get_metrics(sg_ctx* ctx, double* nchi, double* neta, double* volume, doub
 int sg_residual_dev_async(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_dev) {
   if (!ctx || !q_dev || !r_dev) return SG_ERR_ARG;
   SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, l2_dev ? ctx->partial.p : nullptr,
-                         reinterpret_cast<unsigned*>(ctx->counter.p), l2_dev, ctx->stream),
+                         reinterpret_cast<unsigned*>(ctx->counter.p), l2_dev, ctx->stream, ctx->clo),
          "residual launch");
   return SG_OK;
 }
@@ -491,7 +551,8 @@ void* sg_get_stream(sg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 int sg_residual_dev(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_host) {
   if (!ctx || !q_dev || !r_dev) return SG_ERR_ARG;
   SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, ctx->partial.p,
-                         reinterpret_cast<unsigned*>(ctx->counter.p), ctx->l2.p, ctx->stream), "residual launch");
+                         reinterpret_cast<unsigned*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
+         "residual launch");
   double l2[8];
   SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, ctx->nvar * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
   SG_TRY(hipStreamSynchronize(ctx->stream), "residual");
@@ -505,7 +566,8 @@ int sg_residual(sg_ctx* ctx, const double* q_aos, double* r_aos, double* l2) {
   if ((rc = upload_aos(ctx, q_aos, ctx->q.p))) return rc;
   double l2h[8];
   SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, ctx->q.p, ctx->r.p, ctx->partial.p,
-                         reinterpret_cast<unsigned*>(ctx->counter.p), ctx->l2.p, ctx->stream), "residual launch");
+                         reinterpret_cast<unsigned*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
+         "residual launch");
   SG_TRY(hipMemcpyAsync(l2h, ctx->l2.p, ctx->nvar * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
   if ((rc = download_aos(ctx, ctx->r.p, r_aos))) return rc;
   SG_TRY(hipStreamSynchronize(ctx->stream), "residual");
@@ -515,7 +577,8 @@ int sg_residual(sg_ctx* ctx, const double* q_aos, double* r_aos, double* l2) {
 
 int sg_jv_dev(sg_ctx* ctx, const double* q_dev, const double* v_dev, double* jv_dev) {
   if (!ctx || !q_dev || !v_dev || !jv_dev) return SG_ERR_ARG;
-  SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, v_dev, nullptr, jv_dev, ctx->stream), "jv launch");
+  SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, v_dev, nullptr, jv_dev, ctx->stream, nullptr, ctx->clo),
+         "jv launch");
   SG_TRY(hipStreamSynchronize(ctx->stream), "jv");
   return SG_OK;
 }
@@ -525,7 +588,8 @@ int sg_jv(sg_ctx* ctx, const double* q_aos, const double* v_aos, double* jv_aos)
   int rc;
   if ((rc = upload_aos(ctx, q_aos, ctx->q.p))) return rc;
   if ((rc = upload_aos(ctx, v_aos, ctx->v.p))) return rc;
-  SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, ctx->q.p, ctx->v.p, nullptr, ctx->w.p, ctx->stream), "jv launch");
+  SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, ctx->q.p, ctx->v.p, nullptr, ctx->w.p, ctx->stream, nullptr,
+                   ctx->clo), "jv launch");
   if ((rc = download_aos(ctx, ctx->w.p, jv_aos))) return rc;
   SG_TRY(hipStreamSynchronize(ctx->stream), "jv");
   return SG_OK;

@@ -80,6 +80,9 @@ struct sg_ctx {
   sg::DevBuf q, r, v, w;           // SoA work vectors (nvar * plane)
   sg::DevBuf partial, l2, counter;  // counter: one unsigned, kept at 0 between launches
   sg::DevBuf wall_cp, wall_cf, wall_f;
+  sg::DevClosure clo;              // beta closure applied by every residual / J.v launch
+  sg::DevBuf clo_params, clo_w2;   // MLP parameters (FP32) and W2 (BF16), raw storage
+  sg::DevBuf aux;                  // features (5 planes) + beta, for sg_closure_eval
   SolverState* solver = nullptr;   // Newton-Krylov work space (sg_solver.cpp)
   size_t ncell() const { return (size_t)geo.nic * geo.njc; }
   size_t nsoa() const { return (size_t)nvar * g.plane; }

@@ -49,6 +49,20 @@ struct DevBc {
   const int* ghost_map;
 };
 
+// Closure for the SA production multiplier beta:
+//   CLOSURE_FIELD    beta from the per-cell field DevGeom::beta (default)
+//   CLOSURE_ANALYTIC beta = 1 + amp tanh(a . f + a0), FP64, f the local features
+//   CLOSURE_MLP      beta = 1 + MLP(f) on the matrix cores (BF16 operands,
+//                    FP32 accumulation), W1/b1/b2/W3/b3 FP32, W2 BF16
+enum ClosureKind { CLOSURE_FIELD = 0, CLOSURE_ANALYTIC = 1, CLOSURE_MLP = 2 };
+struct DevClosure {
+  int kind = CLOSURE_FIELD;
+  double coef[7] = {0, 0, 0, 0, 0, 0, 0};  // analytic: a[5], a0, amp
+  const float* params = nullptr;           // MLP packed FP32 parameters
+  const void* w2 = nullptr;                // MLP W2 as BF16 [64 out][64 in]
+  double* aux = nullptr;                   // optional output: features (5 planes) and beta (1 plane)
+};
+
 constexpr double kGamma = 1.4;
 
 // SA-noft2 constants (Spalart & Allmaras 1994 as catalogued by NASA TMR),

@@ -17,26 +17,45 @@ size_t residual_partial_count(const DevGeom& g) {
   return (size_t)gr.x * gr.y;
 }
 
+template <class S, bool SA>
+static void launch_cl(dim3 grid, hipStream_t s, const DevGeom& g, const DevPhys& ph, const DevBc& bc, Field<S> fq,
+                      OutField<S> fo, double* partial, const double* beta_dot, const DevClosure& clo) {
+  if constexpr (SA) {
+    if (clo.kind == CLOSURE_ANALYTIC) {
+      hipLaunchKernelGGL((residual_kernel<S, SA, CLOSURE_ANALYTIC>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial,
+                         beta_dot, clo);
+      return;
+    }
+    if (clo.kind == CLOSURE_MLP) {
+      hipLaunchKernelGGL((residual_kernel<S, SA, CLOSURE_MLP>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial,
+                         beta_dot, clo);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((residual_kernel<S, SA, CLOSURE_FIELD>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial,
+                       beta_dot, clo);
+}
+
 hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
-                           double* partial, unsigned* counter, double* l2, hipStream_t s) {
+                           double* partial, unsigned* counter, double* l2, hipStream_t s, const DevClosure& clo) {
   dim3 grid = tile_grid(g);
   Field<double> fq{q};
   OutField<double> fo{r};
   (void)counter;
-  if (sa) hipLaunchKernelGGL((residual_kernel<double, true>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial, nullptr);
-  else hipLaunchKernelGGL((residual_kernel<double, false>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial, nullptr);
+  if (sa) launch_cl<double, true>(grid, s, g, ph, bc, fq, fo, partial, nullptr, clo);
+  else launch_cl<double, false>(grid, s, g, ph, bc, fq, fo, partial, nullptr, DevClosure());
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !partial || !l2) return e;
   return launch_norm_finalize(partial, (int)residual_partial_count(g), 2 * (sa ? 5 : 4), l2, s);
 }
 
 hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,
-                     double* r, double* jv, hipStream_t s, const double* beta_dot) {
+                     double* r, double* jv, hipStream_t s, const double* beta_dot, const DevClosure& clo) {
   dim3 grid = tile_grid(g);
   Field<Dual> fq{q, v};
   OutField<Dual> fo{r, jv};
-  if (sa) hipLaunchKernelGGL((residual_kernel<Dual, true>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, nullptr, beta_dot);
-  else hipLaunchKernelGGL((residual_kernel<Dual, false>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, nullptr, nullptr);
+  if (sa) launch_cl<Dual, true>(grid, s, g, ph, bc, fq, fo, nullptr, beta_dot, clo);
+  else launch_cl<Dual, false>(grid, s, g, ph, bc, fq, fo, nullptr, nullptr, DevClosure());
   return hipGetLastError();
 }
 

@@ -11,10 +11,12 @@ size_t residual_partial_count(const DevGeom& g);
 // (device, 2 nv entries: the L2 norms of R, then of V R) a second kernel
 // finalises the norms.  counter is unused.
 hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
-                           double* partial, unsigned* counter, double* l2, hipStream_t s);
+                           double* partial, unsigned* counter, double* l2, hipStream_t s,
+                           const DevClosure& clo = DevClosure());
 // J.v (+ (dR/dbeta).beta_dot when beta_dot is not null; v may be all zero)
 hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,
-                     double* r, double* jv, hipStream_t s, const double* beta_dot = nullptr);
+                     double* r, double* jv, hipStream_t s, const double* beta_dot = nullptr,
+                     const DevClosure& clo = DevClosure());
 hipError_t launch_norm_finalize(const double* partial, int nblocks, int nv, double* l2, hipStream_t s);
 hipError_t launch_dt(const DevGeom& g, double mu_inf, int nv, const double* q, const double* dsmin, double cfl,
                      double* dt, hipStream_t s);

@@ -20,6 +20,7 @@
 // for MUSCL, one cell around for the viscous stencil) is staged once in LDS
 // as primitives and transport properties.
 #pragma once
+#include "sg_closure.hpp"
 #include "sg_device.hpp"
 #include "sg_scalar.hpp"
 
@@ -252,14 +253,19 @@ __device__ __forceinline__ void transport(const Prim<S>& P, const DevPhys& ph, S
 // SA-noft2 source (production * beta - destruction + cb2 and density-gradient
 // diffusion terms, NASA TMR compressible form) at one cell; the cell
 // gradients are Green-Gauss over its four faces with arithmetic face means.
+// Split as src = beta * P1 - D + rest so that a closure can supply beta.
 template <class S>
-__device__ __forceinline__ S sa_source(S r, S n, S ml, const S* gu, const S* gv, const S* gn, const S* gr, double d,
-                                       S b) {
+struct SaTerms {
+  S P1, D, rest;
+};
+template <class S>
+__device__ __forceinline__ SaTerms<S> sa_terms(S r, S n, S ml, const S* gu, const S* gv, const S* gn, const S* gr,
+                                               double d) {
   const double cw1 = kCb1 / (kKappa * kKappa) + (1.0 + kCb2) / kSigma;
   S nu = sdiv(ml, r);
   S omega = sabs(gv[0] - gu[1]);
   const double k2d2 = kKappa * kKappa * d * d, ik2d2 = frcp(k2d2), id = frcp(d);
-  S P, D;
+  SaTerms<S> t;
   if (val(n) >= 0.0) {
     S chi = sdiv(n, nu), chi3 = chi * chi * chi;
     S fv1 = sdiv(chi3, chi3 + kCv1 * kCv1 * kCv1);
@@ -274,24 +280,30 @@ __device__ __forceinline__ S sa_source(S r, S n, S ml, const S* gu, const S* gv,
     S g2 = gg * gg, g6 = g2 * g2 * g2;
     const double cw3_6 = kCw3 * kCw3 * kCw3 * kCw3 * kCw3 * kCw3;
     S fw = gg * spow(sdiv(S(1.0 + cw3_6), g6 + cw3_6), 1.0 / 6.0);
-    P = b * kCb1 * stil * r * n;
-    D = cw1 * fw * r * (n * id) * (n * id);
+    t.P1 = kCb1 * stil * r * n;
+    t.D = cw1 * fw * r * (n * id) * (n * id);
   } else {
-    P = b * kCb1 * (1.0 - kCt3) * omega * r * n;
-    D = -cw1 * r * (n * id) * (n * id);
+    t.P1 = kCb1 * (1.0 - kCt3) * omega * r * n;
+    t.D = -cw1 * r * (n * id) * (n * id);
   }
   S cb2t = (kCb2 / kSigma) * r * (gn[0] * gn[0] + gn[1] * gn[1]);
   S cross = -(1.0 / kSigma) * (nu + n) * (gr[0] * gn[0] + gr[1] * gn[1]);
-  return P - D + cb2t + cross;
+  t.rest = cb2t + cross;
+  return t;
 }
 
-template <class S, bool SA>
+// CL: closure kind of beta (ClosureKind); CLOSURE_MLP stages the hidden
+// layer of the network (and its tangent) in the same LDS area.
+template <class S, bool SA, int CL>
 __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, DevBc bc, Field<S> q, OutField<S> out,
-                                                       double* partial, const double* beta_dot) {
+                                                       double* partial, const double* beta_dot, DevClosure clo) {
   constexpr int NL = SA ? L_MAX : (L_K + 1);
   constexpr int NV = SA ? 5 : 4;
-  constexpr int NLDS = NL * NPOS > 2 * NV * NT ? NL * NPOS : 2 * NV * NT;
-  __shared__ double lds_raw[NLDS * (sizeof(S) / sizeof(double))];
+  constexpr int NLDS0 = NL * NPOS > 2 * NV * NT ? NL * NPOS : 2 * NV * NT;
+  constexpr int NLDS_D = NLDS0 * (int)(sizeof(S) / sizeof(double));
+  constexpr bool TAN = sizeof(S) > sizeof(double);
+  constexpr int NMLP_D = (CL == CLOSURE_MLP) ? (TAN ? 2 : 1) * (NT / 64) * 64 * kHld / 4 : 0;
+  __shared__ double lds_raw[NLDS_D > NMLP_D ? NLDS_D : NMLP_D];
   S* lds = reinterpret_cast<S*>(lds_raw);
   __shared__ double red[NT / 64][2 * NV];
 
@@ -465,24 +477,63 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
   const int ci = i0 + tx, cj = j0 + ty;
   const bool owns = tx < OX && ty < OY && ci < nic && cj < njc;
   S src(0.0);
-  if (SA && owns && ph.viscous) {
-    const int x = tx + 2, y = ty + 2;
-    const double ex = ncx(ci + 1, cj), ey = ncy(ci + 1, cj), wx = ncx(ci, cj), wy = ncy(ci, cj);
-    const double nx_ = nex(ci, cj + 1), ny_ = ney(ci, cj + 1), sx = nex(ci, cj), sy = ney(ci, cj);
-    const double iV = frcp(vol(ci, cj));
-    auto cgrad = [&](int k, S* o) {
-      S c = at(k, x, y);
-      S fe = 0.5 * (c + at(k, x + 1, y)), fw = 0.5 * (c + at(k, x - 1, y));
-      S fn = 0.5 * (c + at(k, x, y + 1)), fs = 0.5 * (c + at(k, x, y - 1));
-      o[0] = (fe * ex - fw * wx + fn * nx_ - fs * sx) * iV;
-      o[1] = (fe * ey - fw * wy + fn * ny_ - fs * sy) * iV;
-    };
-    S gu[2], gv[2], gn[2], gr[2];
-    cgrad(L_U, gu); cgrad(L_V, gv); cgrad(L_N, gn); cgrad(L_R, gr);
+  if (SA && ph.viscous) {  // uniform over the block
     const size_t cidx = (size_t)cj * g.ldi + ci;
-    S bcell(g.beta[cidx]);
-    if (beta_dot) bcell = bcell + beta_tangent<S>(beta_dot[cidx]);
-    src = sa_source<S>(at(L_R, x, y), at(L_N, x, y), at(L_ML, x, y), gu, gv, gn, gr, g.wd[cidx], bcell);
+    SaTerms<S> st{S(0.0), S(0.0), S(0.0)};
+    S feat[kNFeat];
+    if (owns) {
+      const int x = tx + 2, y = ty + 2;
+      const double ex = ncx(ci + 1, cj), ey = ncy(ci + 1, cj), wx = ncx(ci, cj), wy = ncy(ci, cj);
+      const double nx_ = nex(ci, cj + 1), ny_ = ney(ci, cj + 1), sx = nex(ci, cj), sy = ney(ci, cj);
+      const double iV = frcp(vol(ci, cj));
+      auto cgrad = [&](int k, S* o) {
+        S c = at(k, x, y);
+        S fe = 0.5 * (c + at(k, x + 1, y)), fw = 0.5 * (c + at(k, x - 1, y));
+        S fn = 0.5 * (c + at(k, x, y + 1)), fs = 0.5 * (c + at(k, x, y - 1));
+        o[0] = (fe * ex - fw * wx + fn * nx_ - fs * sx) * iV;
+        o[1] = (fe * ey - fw * wy + fn * ny_ - fs * sy) * iV;
+      };
+      S gu[2], gv[2], gn[2], gr[2];
+      cgrad(L_U, gu); cgrad(L_V, gv); cgrad(L_N, gn); cgrad(L_R, gr);
+      const S r = at(L_R, x, y), n = at(L_N, x, y), ml = at(L_ML, x, y);
+      st = sa_terms<S>(r, n, ml, gu, gv, gn, gr, g.wd[cidx]);
+      if (CL != CLOSURE_FIELD || clo.aux) {
+        S gp[2];
+        cgrad(L_P, gp);
+        sa_features<S>(r, at(L_U, x, y), at(L_V, x, y), n, sdiv(ml, r), gu, gv, gp, g.wd[cidx], st.P1, st.D, ph, feat);
+      }
+    }
+    S bcell(1.0);
+    if (CL == CLOSURE_FIELD) {
+      if (owns) {
+        bcell = S(g.beta[cidx]);
+        if (beta_dot) bcell = bcell + beta_tangent<S>(beta_dot[cidx]);
+      }
+    } else if (CL == CLOSURE_ANALYTIC) {
+      if (owns) bcell = closure_analytic<S>(clo, feat);
+    } else {
+      float xv[kNFeat], xt[kNFeat];
+#pragma unroll
+      for (int k = 0; k < kNFeat; ++k) {
+        xv[k] = owns ? (float)val(feat[k]) : 0.f;
+        xt[k] = owns ? (float)tan_(feat[k]) : 0.f;
+      }
+      __syncthreads();  // staged cells no longer read: reuse LDS for the hidden layer
+      __bf16* hs = reinterpret_cast<__bf16*>(lds_raw) + (tid >> 6) * 64 * kHld;
+      __bf16* ht = hs + (NT / 64) * 64 * kHld;
+      float y, yt;
+      mlp_wave<TAN>(clo.params, reinterpret_cast<const __bf16*>(clo.w2), xv, xt, hs, ht, y, yt);
+      bcell = S(1.0 + (double)y);
+      if (TAN) bcell = bcell + beta_tangent<S>((double)yt);
+    }
+    if (owns) {
+      src = bcell * st.P1 - st.D + st.rest;
+      if (clo.aux) {
+#pragma unroll
+        for (int k = 0; k < kNFeat; ++k) clo.aux[k * g.plane + cidx] = val(feat[k]);
+        clo.aux[kNFeat * g.plane + cidx] = val(bcell);
+      }
+    }
   }
 
   // ---- exchange face fluxes through LDS (reusing the staging area)

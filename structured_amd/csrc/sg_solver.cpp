@@ -132,7 +132,8 @@ int apply_operator(sg_ctx* ctx, const double* z, double* w) {
   if (S->transpose)
     SG_TRY(jt_gather(ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->nvar, S->blocks.p, z, S->jz.p, ctx->stream), "jt");
   else
-    SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, S->qop.p, z, nullptr, S->jz.p, ctx->stream), "jv");
+    SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, S->qop.p, z, nullptr, S->jz.p, ctx->stream, nullptr, ctx->clo),
+           "jv");
   SG_TRY(shift_minus(z, S->idt.p, S->jz.p, w, ctx->g.plane, ctx->nvar, ctx->stream), "shift");
   return SG_OK;
 }
@@ -144,7 +145,8 @@ int extract_jacobian(sg_ctx* ctx) {
     for (int b = 0; b < 5; ++b)
       for (int m = 0; m < nv; ++m) {
         SG_TRY(seed_colour(nic, njc, ldi, nv, a, b, m, S->seed.p, ctx->stream), "seed");
-        SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, S->qop.p, S->seed.p, nullptr, S->jz.p, ctx->stream), "jv");
+        SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, S->qop.p, S->seed.p, nullptr, S->jz.p, ctx->stream, nullptr,
+                         ctx->clo), "jv");
         SG_TRY(extract_blocks(nic, njc, ldi, nv, a, b, m, S->jz.p, S->blocks.p, ctx->stream), "extract");
       }
   return SG_OK;
@@ -373,14 +375,15 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
   int lin_total = 0;
   double ser_ref = 0.0, cfl_factor = 1.0, last_alpha = 1.0;
   int recoveries = 0;
-  bool converged = false;
+  bool converged = false, stalled = false;
+  std::vector<double> lmax_hist;
   int it = 0;
   for (;; ++it) {
     Timer tr;
     SG_TRY(launch_dt(ctx->g, ctx->ph.mu_inf, nv, q, ctx->dsmin.p, cfl, S->dtv.p, ctx->stream), "dt");
     SG_TRY(invert_dt(S->dtv.p, S->idt.p, ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->stream), "invert");
     SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q, S->rhs.p, ctx->partial.p, nullptr, ctx->l2.p,
-                           ctx->stream), "residual");
+                           ctx->stream, ctx->clo), "residual");
     SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
     SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
     t_res += tr.s();
@@ -406,6 +409,14 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     }
     if (lmax < o->tolerance) { converged = true; break; }
     if (it >= o->iteration_max) break;
+    lmax_hist.push_back(lmax);
+    if (o->stall_window > 0 && (int)lmax_hist.size() > 2 * o->stall_window) {
+      const size_t w = (size_t)o->stall_window, m = lmax_hist.size();
+      const double before = *std::min_element(lmax_hist.begin(), lmax_hist.end() - w);
+      const double recent = *std::min_element(lmax_hist.end() - w, lmax_hist.end());
+      if (recent < 1e3 * o->tolerance && recent > 0.5 * before) { stalled = true; break; }
+      (void)m;
+    }
     Timer tj;
     S->pc_boost = (o->pc_cfl_max > 0.0 && cfl > o->pc_cfl_max) ? cfl / o->pc_cfl_max : 1.0;
     SG_TRY(hipMemcpyAsync(S->qop.p, q, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
@@ -452,6 +463,7 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
   int rc2 = download(ctx, q, q_aos);
   R.iterations = it;
   R.converged = converged ? 1 : 0;
+  R.stalled = stalled ? 1 : 0;
   R.linear_iterations = lin_total;
   for (int k = 0; k < nv && k < 5; ++k) {
     R.l2_final[k] = l2[k];
@@ -488,6 +500,8 @@ int adjoint_prepare(sg_ctx* ctx, const double* q_dev) {
   if (!ctx->bcc.local)
     return sg_fail(ctx, SG_ERR_ARG, "adjoint products need boundary conditions whose ghost cells depend on nearby cells "
                                     "only (no wake or wide periodic boundaries)");
+  if (ctx->clo.kind != sg::CLOSURE_FIELD)
+    return sg_fail(ctx, SG_ERR_ARG, "adjoint products with respect to the beta field need the field closure");
   int rc;
   if (!ctx->solver && (rc = ensure(ctx, 30))) return rc;
   SolverState* S = ctx->solver;

@@ -1,0 +1,137 @@
+"""Field inversion and machine learning of the SA production multiplier.
+
+1. Synthetic truth: beta* = 1 + amp tanh(c . f + c0) on the local features
+   f (the analytic closure, FP64); its converged wall Cf is the target.
+2. Field inversion: adjoint-driven L-BFGS from beta = 1 (sg_invert).
+3. Training data: (features, beta_inv - 1) at the inverted solution.
+4. The closure network, trained on the GPU (sg_mlp_train).
+5. Prediction: a forward solve with the network fused into the SA source.
+
+Everything runs through the C ABI (structured_amd.lib); this module only
+sequences the calls and keeps the timings.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import cases, lib
+
+# Newton-Krylov settings of the SA solves (pseudo-transient continuation with
+# switched evolution relaxation on the flux-balance norm).
+NEWTON = dict(cfl=5.0, ser=1, ser_exponent=1.5, cfl_max=1e10, gmres_rtol=1e-4, precond=4, pc_cfl_max=300.0,
+              gmres_restart=40, gmres_maxit=200, iteration_max=400, norm_weighted=1, tolerance=1e-12)
+# warm starts (a converged state of a nearby problem): the residual first rises
+# while the boundary layer adapts, so the pseudo-time step starts larger; much
+# larger starting CFL numbers outrun the preconditioner (its factorisation is
+# capped at pc_cfl_max) and stall GMRES
+WARM = dict(NEWTON, cfl=100.0)
+RESOLVE = dict(WARM, iteration_max=150)
+# the BF16 closure network makes beta a piecewise-constant function of the
+# state at its rounding points, which sets a floor on the flux-balance norm
+# (measured ~1.5e-11 on the 65x49 plate against 2e-12 for plain SA)
+MLP_TOL = 1e-10
+MLP_STALL = 8
+ADJOINT = dict(restart=60, maxit=3000, rtol=1e-8, precond=4, pc_cfl=300.0)
+
+
+def truth_coef(seed: int = 0, amp: float = 0.3) -> np.ndarray:
+    """Seeded coefficients of beta* = 1 + amp tanh(c . f + c0)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-2.0, 2.0, 5)
+    c0 = rng.uniform(-0.5, 0.5)
+    return np.r_[c, c0, amp]
+
+
+def bump_case(ni, nj, height, width, re=3e6, dy1=1e-6):
+    return cases.sa_bump(ni, nj, height=height, width=width, dy1=dy1, re=re)
+
+
+def timed_solve(eq, q0, **opts):
+    t = time.perf_counter()
+    q, rep, hist = eq.solve(q0, **opts)
+    return q, rep, time.perf_counter() - t
+
+
+def closure_solve_ok(rep, floor=1e-9):
+    """Converged, or stalled at the closure's floor below `floor`."""
+    return bool(rep["converged"]) or (bool(rep["stalled"]) and max(rep["l2w_final"]) < floor)
+
+
+def cf_error(cf, ref):
+    """Relative L2 error of a wall Cf distribution."""
+    return float(np.linalg.norm(cf - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def run_case(case, coef, invert_iterations=50, newton=None, verbose=False):
+    """Truth, baseline and inversion on one case; returns a dict with the
+    states, beta fields, wall Cf and timings."""
+    newton = dict(NEWTON, **(newton or {}))
+    eq = lib.Equation(case)
+    out = {"case": case.name}
+    q_sa, rep, t = timed_solve(eq, case.freestream_state(), **newton)
+    out.update(q_sa=q_sa, sa_report=rep, sa_s=t)
+    eq.set_closure(1, coef=coef)
+    q_true, rep, t = timed_solve(eq, q_sa, **dict(newton, cfl=WARM["cfl"]))
+    out.update(q_true=q_true, true_report=rep, true_s=t)
+    feats_true, beta_true = eq.closure_eval(q_true)
+    _, _, cf_true, _ = eq.wall(q_true)
+    eq.set_closure(0)
+    _, _, cf_sa, _ = eq.wall(q_sa)
+    out.update(beta_true=beta_true, features_true=feats_true, cf_true=cf_true, cf_sa=cf_sa)
+    if invert_iterations <= 0:
+        eq.close()
+        return out
+    t0 = time.perf_counter()
+    q_inv, beta_inv, irep, jh = eq.invert(q_sa, cf_true, iterations=invert_iterations, newton=dict(RESOLVE),
+                                          adjoint=ADJOINT)
+    out.update(q_inv=q_inv, beta_inv=beta_inv, inv_report=irep, inv_hist=jh, inv_s=time.perf_counter() - t0)
+    _, _, cf_inv, _ = eq.wall(q_inv)
+    feats, _ = eq.closure_eval(q_inv)
+    out.update(cf_inv=cf_inv, features=feats)
+    if verbose:
+        irep = out["inv_report"]
+        print("%s: SA %.2fs (%d it), truth %.2fs, inversion %.2fs (%d it, J %.3e -> %.3e); "
+              "Cf err SA %.3e inv %.3e" % (case.name, out["sa_s"], out["sa_report"]["iterations"], out["true_s"],
+                                           out["inv_s"], irep["iterations"], irep["J0"], irep["J"],
+                                           cf_error(cf_sa, cf_true), cf_error(cf_inv, cf_true)), flush=True)
+    eq.close()
+    return out
+
+
+def training_set(results, wall_band=None):
+    """Stack (features, beta_inv - 1) over the inverted cases.  wall_band:
+    keep only cells with f1 (log wall-distance Reynolds number) above it."""
+    X, T = [], []
+    for r in results:
+        f = r["features"].reshape(-1, 5)
+        t = (r["beta_inv"] - 1.0).reshape(-1)
+        if wall_band is not None:
+            m = f[:, 1] > wall_band
+            f, t = f[m], t[m]
+        X.append(f)
+        T.append(t)
+    return np.concatenate(X).astype(np.float32), np.concatenate(T).astype(np.float32)
+
+
+def train(X, T, epochs=200, batch=65536, lr=1e-3, seed=0):
+    net = lib.Mlp(max_rows=min(batch, len(X)), seed=seed)
+    t = time.perf_counter()
+    hist = net.train(X, T, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
+    return net, hist, time.perf_counter() - t
+
+
+def predict(case, net, q0=None, newton=None):
+    """Forward solve with the network fused into the SA source; returns
+    (q, report, seconds, wall Cf)."""
+    o = dict(NEWTON, tolerance=MLP_TOL, stall_window=MLP_STALL)
+    if q0 is not None:
+        o["cfl"] = WARM["cfl"]
+    o.update(newton or {})
+    eq = lib.Equation(case)
+    eq.set_closure_mlp(net)
+    q, rep, t = timed_solve(eq, case.freestream_state() if q0 is None else q0, **o)
+    _, _, cf, _ = eq.wall(q)
+    eq.close()
+    return q, rep, t, cf

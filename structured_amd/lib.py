@@ -264,3 +264,117 @@ class Equation:
             self._check(rc)
         return gb, F.value, {"iterations": rep.iterations, "relres": rep.relres, "wall_s": rep.wall_s,
                              "converged": rc == 0}
+
+    def invert(self, q, cf_target, beta=None, iterations=50, memory=10, reg=0.0, step0=0.2, beta_min=0.0,
+               beta_max=5.0, max_backtracks=6, newton=None, adjoint=None):
+        """Adjoint-driven L-BFGS field inversion of beta from the converged
+        state q at the initial beta (default 1); returns (q, beta, report,
+        J history)."""
+        q = _f64(q, self.shape).copy()
+        b = np.ones((self.nic, self.njc)) if beta is None else _f64(beta, (self.nic, self.njc)).copy()
+        adj = dict(restart=60, maxit=2000, rtol=1e-10, precond=4, reserved=0, pc_cfl=300.0)
+        adj.update(adjoint or {})
+        o = abi.InversionOpts(iterations=iterations, memory=memory, reg=reg, step0=step0, beta_min=beta_min,
+                              beta_max=beta_max, max_backtracks=max_backtracks, reserved=0,
+                              newton=abi.newton_opts(**(newton or {})), adjoint=abi.AdjointOpts(**adj))
+        rep = abi.InversionReport()
+        hist = np.full(iterations + 1, np.nan)
+        self._check(load().sg_invert(self.h, _p(q), _p(_f64(cf_target)), _p(b), C.byref(o), C.byref(rep),
+                                     _p(hist)))
+        report = {f: getattr(rep, f) for f, _ in abi.InversionReport._fields_}
+        return q, b, report, hist[: rep.iterations + 1]
+
+    # ---- beta closures (no counterpart in the reference)
+    CLOSURE_FIELD, CLOSURE_ANALYTIC, CLOSURE_MLP = 0, 1, 2
+
+    def set_closure(self, kind, coef=None, params=None):
+        """kind 0: beta field (sg_set_beta); 1: beta = 1 + c6 tanh(c[:5].f + c5);
+        2: beta = 1 + MLP(f) with params (SG_MLP_NPARAM float32)."""
+        c = None if coef is None else _f64(coef, (7,))
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        if p is not None and p.size != Mlp.NPARAM:
+            raise ValueError("MLP parameter vector must have %d entries" % Mlp.NPARAM)
+        self._check(load().sg_set_closure(self.h, int(kind), _p(c), _p(p)))
+
+    def set_closure_mlp(self, net):
+        self._check(load().sg_set_closure_mlp(self.h, net.h))
+
+    def closure_eval(self, q):
+        """Local features [nic][njc][5] and the closure's beta [nic][njc] at q."""
+        q = _f64(q, self.shape)
+        f = np.empty((self.nic, self.njc, 5))
+        b = np.empty((self.nic, self.njc))
+        self._check(load().sg_closure_eval(self.h, _p(q), _p(f), _p(b)))
+        return f, b
+
+
+class Mlp:
+    """Closure network 5 -> 64 -> 64 -> 1 (tanh), trained on the GPU
+    (sg_mlp_* entry points).  Parameters: float32 vector
+    W1[64][5] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3."""
+    NPARAM = 4609
+    SLICES = {"W1": (0, (64, 5)), "b1": (320, (64,)), "W2": (384, (64, 64)), "b2": (4480, (64,)),
+              "W3": (4544, (64,)), "b3": (4608, ())}
+
+    def __init__(self, max_rows=65536, seed=0):
+        lib = load()
+        lib.sg_mlp_last_error.restype = C.c_char_p
+        lib.sg_mlp_last_error.argtypes = [C.c_void_p]
+        lib.sg_mlp_destroy.argtypes = [C.c_void_p]
+        lib.sg_mlp_destroy.restype = None
+        h = C.c_void_p()
+        rc = lib.sg_mlp_create(C.byref(h), int(max_rows), C.c_ulonglong(seed))
+        if rc != 0:
+            raise SgError(rc, "sg_mlp_create failed")
+        self.h = h
+        self.max_rows = max_rows
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().sg_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SgError(rc, load().sg_mlp_last_error(self.h).decode())
+
+    @property
+    def params(self):
+        p = np.empty(self.NPARAM, dtype=np.float32)
+        self._check(load().sg_mlp_get_params(self.h, _p(p)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.NPARAM:
+            raise ValueError("expected %d parameters" % self.NPARAM)
+        self._check(load().sg_mlp_set_params(self.h, _p(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 5)
+        y = np.empty(x.shape[0], dtype=np.float32)
+        self._check(load().sg_mlp_forward(self.h, x.shape[0], _p(x), _p(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 5)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        loss = C.c_float()
+        g = np.empty(self.NPARAM, dtype=np.float32)
+        self._check(load().sg_mlp_loss_grad(self.h, x.shape[0], _p(x), _p(t), C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def train(self, x, t, epochs=1, batch=65536, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 5)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        hist = np.zeros(max(epochs, 1), dtype=np.float32)
+        self._check(load().sg_mlp_train(self.h, x.shape[0], _p(x), _p(t), int(epochs), int(batch), C.c_float(lr),
+                                        C.c_ulonglong(seed), _p(hist)))
+        return hist[:epochs]

@@ -164,3 +164,22 @@ class RefRun:
             out.append(raw[off:off + n].reshape(shp))
             off += n
         return out
+
+
+def set_closure(kind, coef=None, params=None):
+    """Closure of beta for the following oracle calls (0 field, 1 analytic,
+    2 MLP)."""
+    c = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+    p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+    lib().og_set_closure(int(kind), None if c is None else abi.dptr(c),
+                         None if p is None else p.ctypes.data_as(C.c_void_p))
+
+
+def closure_eval(case, q, wd=None, beta=None):
+    """(features [nic][njc][5], beta [nic][njc]) at q."""
+    xv, yv, fs, bcs, o = _common(case)
+    q, wd, beta = _arr(q), _arr(wd), _arr(beta)
+    out = np.zeros((case.nic, case.njc, 6))
+    lib().og_closure_eval(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs), C.byref(o),
+                          abi.dptr(wd), abi.dptr(beta), abi.dptr(q), abi.dptr(out))
+    return out[..., :5], out[..., 5]

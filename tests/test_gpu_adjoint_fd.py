@@ -43,7 +43,7 @@ def test_adjoint_gradient_matches_central_differences():
     q, rep, _ = eq.solve(q1, **NEWTON)
     assert rep["converged"], rep
     g, F, arep = eq.adjoint_gradient(q, cf_target, rtol=1e-12, maxit=2000, restart=60, precond=4, pc_cfl=300.0)
-    print("F", F, "adjoint", arep)
+    print("F", F, "adjoint", arep, flush=True)
     assert arep["converged"]
     rng = np.random.default_rng(7)
     # 16 cells in the region the wall Cf is sensitive to (near the wall)
@@ -59,6 +59,7 @@ def test_adjoint_gradient_matches_central_differences():
             eq.set_beta(b)
             qp, r2, _ = eq.solve(q, **RESOLVE)  # to the roundoff floor
             vals.append(eq.objective_cf(qp, cf_target, grad=False)[0])
+            print("  re-solve", (i, j), sgn * eps, "newton", r2["iterations"], flush=True)
         return (vals[0] - vals[1]) / (2 * eps)
 
     gmax = np.abs(g).max()
@@ -68,7 +69,7 @@ def test_adjoint_gradient_matches_central_differences():
         rel = abs(g[i, j] - fd) / max(abs(fd), 1e-2 * gmax)
         worst = max(worst, rel)
         rels.append(rel)
-        print("cell", (i, j), "adjoint %.10e fd %.10e rel %.2e" % (g[i, j], fd, rel))
+        print("cell", (i, j), "adjoint %.10e fd %.10e rel %.2e" % (g[i, j], fd, rel), flush=True)
     eq.set_beta(beta)
     # Most cells agree to ~1e-8; a cell whose +-eps perturbation moves a face
     # across a kink of the discretisation (the Roe |u_n| at near-wall faces
