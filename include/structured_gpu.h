@@ -213,6 +213,11 @@ typedef struct sg_newton_opts {
    * last stall_window steps (a floor set by rounding or by a
    * piecewise-constant closure); the report then has stalled = 1. */
   int stall_window;
+  /* solver.lhs_order (src/utils/config.cpp): 0 = the residual's order; 1 =
+   * the preconditioner factors the Jacobian of the first-order residual
+   * (src/solver/solver.cpp:92-101); the Krylov operator stays exact. */
+  int lhs_order;
+  int reserved3;
 } sg_newton_opts;
 
 typedef struct sg_solve_report {

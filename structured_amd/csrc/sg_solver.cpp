@@ -46,6 +46,7 @@ struct SolverState {
                              // 5 level-scheduled natural-order ILU(0)
   int sweeps = 1;
   double pc_boost = 1.0;     // preconditioner uses idt * pc_boost
+  int lhs_order = 0;         // > 0: preconditioner blocks from the residual at this MUSCL order
   bool transpose = false;    // adjoint solves: operator and preconditioner transposed
   double rtol = 1e-3;
   int maxit = 60;
@@ -138,14 +139,19 @@ int apply_operator(sg_ctx* ctx, const double* z, double* w) {
   return SG_OK;
 }
 
-int extract_jacobian(sg_ctx* ctx) {
+// Blocks of J at qop by 5 x 5 colouring of J.v.  order > 0 evaluates the
+// residual at that MUSCL order (the reference's lhs_order: a first-order
+// Jacobian for the implicit operator, solver.cpp:92-101).
+int extract_jacobian(sg_ctx* ctx, int order = 0) {
+  sg::DevPhys ph = ctx->ph;
+  if (order > 0) ph.order = order;
   SolverState* S = ctx->solver;
   const int nic = ctx->geo.nic, njc = ctx->geo.njc, ldi = ctx->g.ldi, nv = ctx->nvar;
   for (int a = 0; a < 5; ++a)
     for (int b = 0; b < 5; ++b)
       for (int m = 0; m < nv; ++m) {
         SG_TRY(seed_colour(nic, njc, ldi, nv, a, b, m, S->seed.p, ctx->stream), "seed");
-        SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, S->qop.p, S->seed.p, nullptr, S->jz.p, ctx->stream, nullptr,
+        SG_TRY(launch_jv(ctx->g, ph, ctx->bc, ctx->sa, S->qop.p, S->seed.p, nullptr, S->jz.p, ctx->stream, nullptr,
                          ctx->clo), "jv");
         SG_TRY(extract_blocks(nic, njc, ldi, nv, a, b, m, S->jz.p, S->blocks.p, ctx->stream), "extract");
       }
@@ -177,7 +183,7 @@ int build_preconditioner(sg_ctx* ctx) {
   SolverState* S = ctx->solver;
   if (S->precond == 0) return SG_OK;
   int rc;
-  if ((rc = extract_jacobian(ctx))) return rc;
+  if ((rc = extract_jacobian(ctx, S->lhs_order))) return rc;
   return factor_preconditioner(ctx, S->idt.p);
 }
 
@@ -419,6 +425,7 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     }
     Timer tj;
     S->pc_boost = (o->pc_cfl_max > 0.0 && cfl > o->pc_cfl_max) ? cfl / o->pc_cfl_max : 1.0;
+    S->lhs_order = o->lhs_order;
     SG_TRY(hipMemcpyAsync(S->qop.p, q, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
     if ((rc = build_preconditioner(ctx))) return rc;
     SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
