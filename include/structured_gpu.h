@@ -115,8 +115,9 @@ typedef struct sg_adjoint_opts {
   int restart;     /* GMRES restart length                                  */
   int maxit;       /* total Krylov iterations                               */
   double rtol;     /* relative residual tolerance                           */
-  int precond;     /* 0 none, 4 five-colour ILU(0), 5 natural-order ILU(0)  */
-  int reserved;
+  int precond;     /* 0 none, 4 five-colour ILU(0), 5 natural-order ILU(0),
+                      6 chunked line ILU(0)                                 */
+  int lhs_order;   /* 1: the preconditioner factors the first-order Jacobian */
   double pc_cfl;   /* the preconditioner factors I/dt - J at this CFL        */
 } sg_adjoint_opts;
 typedef struct sg_adjoint_report {
@@ -170,11 +171,17 @@ void* sg_get_stream(sg_ctx* ctx);
  *      at most `maxit` iterations, relative tolerance `rtol`). */
 int sg_ls_preallocate(sg_ctx* ctx, int restart, int precond, int maxit, double rtol);
 int sg_ls_set_lhs(sg_ctx* ctx, const double* q_aos, const double* dt_aos);
+/* Preconditioner settings for the following set_lhs (after preallocate):
+ * lhs_order as in sg_newton_opts, the factor (>= 1) by which the
+ * preconditioner's 1/dt is larger than the operator's, and the chunk width of
+ * precond 6 (8, 16 or 32; 0 keeps the current one). */
+int sg_ls_set_pc(sg_ctx* ctx, int lhs_order, double pc_dt_factor, int pc_width);
 int sg_ls_set_rhs(sg_ctx* ctx, const double* rhs_aos);
 int sg_ls_solve(sg_ctx* ctx, double* dq_aos, int* iters, double* relres);
 int sg_ls_solve_and_update(sg_ctx* ctx, double* q_aos, double under_relaxation);
-/* z = M^-1 v with the preconditioner built by the last set_lhs. */
-int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos);
+/* z = M^-1 v (transpose = 0) or M^-T v (transpose = 1, preconditioners 0,
+ * 4, 5 and 6) with the preconditioner built by the last set_lhs. */
+int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos, int transpose);
 
 /* ---- nonlinear solve: Solver::solve / Solver::step, src/solver/solver.cpp:52-236.
  *      Ramp semantics as the reference (after step `counter`, if
@@ -217,7 +224,7 @@ typedef struct sg_newton_opts {
    * the preconditioner factors the Jacobian of the first-order residual
    * (src/solver/solver.cpp:92-101); the Krylov operator stays exact. */
   int lhs_order;
-  int reserved3;
+  int pc_width;  /* chunk width of precond 6: 8, 16 or 32 (0: 16) */
 } sg_newton_opts;
 
 typedef struct sg_solve_report {

@@ -51,7 +51,7 @@ class NewtonOpts(C.Structure):
                 ("gmres_restart", C.c_int), ("gmres_maxit", C.c_int), ("gmres_rtol", C.c_double), ("precond", C.c_int),
                 ("ser", C.c_int), ("ser_exponent", C.c_double), ("verbose", C.c_int), ("pc_cfl_max", C.c_double),
                 ("norm_weighted", C.c_int), ("stall_window", C.c_int),
-                ("lhs_order", C.c_int), ("reserved3", C.c_int)]
+                ("lhs_order", C.c_int), ("pc_width", C.c_int)]
 
 
 class SolveReport(C.Structure):
@@ -67,7 +67,7 @@ NEWTON_DEFAULTS = dict(cfl=1.0, cfl_ramp=0, cfl_ramp_iteration=20, cfl_ramp_expo
                        under_relaxation=1.0, under_relaxation_ramp=0, under_relaxation_ramp_iteration=20,
                        under_relaxation_ramp_exponent=1.1, iteration_max=1, tolerance=1e-10, gmres_restart=30,
                        gmres_maxit=60, gmres_rtol=1e-3, precond=4, ser=0, ser_exponent=1.0, verbose=0,
-                       pc_cfl_max=0.0, norm_weighted=0, stall_window=0, lhs_order=0, reserved3=0)
+                       pc_cfl_max=0.0, norm_weighted=0, stall_window=0, lhs_order=0, pc_width=0)
 
 
 def newton_opts(**kw):
@@ -81,7 +81,7 @@ def newton_opts(**kw):
 
 class AdjointOpts(C.Structure):
     _fields_ = [("restart", C.c_int), ("maxit", C.c_int), ("rtol", C.c_double), ("precond", C.c_int),
-                ("reserved", C.c_int), ("pc_cfl", C.c_double)]
+                ("lhs_order", C.c_int), ("pc_cfl", C.c_double)]
 
 
 class AdjointReport(C.Structure):

@@ -1,0 +1,326 @@
+// Chunked line block ILU(0) of A = I/dt - J on the 13-point stencil.
+//
+// The i-range is cut into chunks of W columns.  Inside a chunk the cells are
+// ordered column by column with j fastest (o = (i - i0) njc + j), so every
+// wall-normal line is contiguous and its banded part is factored exactly;
+// couplings between chunks are dropped (block Jacobi over chunks).  In that
+// order the lower neighbours of (i, j) are (-2,0), (-1,-1), (-1,0), (-1,+1),
+// (0,-2), (0,-1) and all of them lie on an earlier level L = j + 2 (i - i0);
+// a chunk has njc + 2W - 2 levels of at most W cells.  One work-group per
+// chunk walks the levels with a barrier between them.
+//
+// Factors are stored per chunk in level order, one record per level:
+// rec[((chunk * nlev + L) * NSLOT * B + s * B + e) * W + c] for block entry e
+// of slot s of the cell in column c of the chunk (row j = L - 2c): slot 0 the
+// inverse of the factored diagonal block, lower slots L_rn (unit lower
+// factor), upper slots U_ru.  Cells of a level sit next to each other, so a
+// level's record is read with coalesced loads.
+#include <hip/hip_runtime.h>
+
+#include "sg_linalg.hpp"
+
+namespace sg {
+
+__constant__ int kCDi[NSLOT] = {0, 0, 0, -1, 1, -1, 1, -1, 1, 0, 0, -2, 2};
+__constant__ int kCDj[NSLOT] = {0, -1, 1, 0, 0, -1, -1, 1, 1, -2, 2, 0, 0};
+// lower / upper slots in increasing chunk order o
+__constant__ int kCLow[6] = {11, 5, 3, 7, 9, 1};
+__constant__ int kCUp[6] = {2, 10, 6, 4, 8, 12};
+
+__device__ __forceinline__ int cslot(int di, int dj) {
+  for (int s = 0; s < NSLOT; ++s)
+    if (kCDi[s] == di && kCDj[s] == dj) return s;
+  return -1;
+}
+
+template <int NV>
+__device__ void cinv(const double* A, double* Ainv) {
+  double M[NV][2 * NV];
+  for (int r = 0; r < NV; ++r)
+    for (int c = 0; c < NV; ++c) { M[r][c] = A[r * NV + c]; M[r][NV + c] = (r == c) ? 1.0 : 0.0; }
+  for (int p = 0; p < NV; ++p) {
+    int piv = p;
+    double best = fabs(M[p][p]);
+    for (int r = p + 1; r < NV; ++r)
+      if (fabs(M[r][p]) > best) { best = fabs(M[r][p]); piv = r; }
+    if (piv != p)
+      for (int c = 0; c < 2 * NV; ++c) { double t = M[p][c]; M[p][c] = M[piv][c]; M[piv][c] = t; }
+    const double ip = 1.0 / M[p][p];
+    for (int c = 0; c < 2 * NV; ++c) M[p][c] *= ip;
+    for (int r = 0; r < NV; ++r) {
+      if (r == p) continue;
+      const double f = M[r][p];
+      if (f != 0.0)
+        for (int c = 0; c < 2 * NV; ++c) M[r][c] -= f * M[p][c];
+    }
+  }
+  for (int r = 0; r < NV; ++r)
+    for (int c = 0; c < NV; ++c) Ainv[r * NV + c] = M[r][NV + c];
+}
+
+template <int NV, int W>
+__global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int njc, int ldi, int nlev, const double* idt,
+                                                                  const double* blocks, double* rec) {
+  constexpr int B = NV * NV, RS = NSLOT * B * W;
+  __shared__ double A[W][NSLOT][B];
+  __shared__ double Lb[W][B];
+  const int chunk = blockIdx.x, i0 = chunk * W, wk = min(W, nic - i0);
+  const size_t plane = (size_t)njc * ldi;
+  const int t = threadIdx.x, c = t / B, e = t % B, r = e / NV, q = e % NV;
+  double* R = rec + (size_t)chunk * nlev * RS;
+  auto inside = [&](int cc, int jj) { return cc >= 0 && cc < wk && jj >= 0 && jj < njc; };
+  for (int L = 0; L < nlev; ++L) {
+    const int j = L - 2 * c, i = i0 + c;
+    const bool valid = inside(c, j);
+    const size_t cell = valid ? (size_t)j * ldi + i : 0;
+    for (int s = 0; s < NSLOT; ++s) {
+      double a = 0.0;
+      if (valid && inside(c + kCDi[s], j + kCDj[s])) {
+        a = -blocks[(size_t)(s * B + e) * plane + cell];
+        if (s == 0 && r == q) a += idt[cell];
+      }
+      A[c][s][e] = a;
+    }
+    __syncthreads();
+    for (int qi = 0; qi < 6; ++qi) {
+      const int s = kCLow[qi], di = kCDi[s], dj = kCDj[s];
+      const int cn = c + di, jn = j + dj;
+      const bool nok = valid && inside(cn, jn);
+      const double* Rn = R + (size_t)(nok ? jn + 2 * cn : 0) * RS;
+      double l = 0.0;
+      if (nok)
+        for (int k = 0; k < NV; ++k) l += A[c][s][r * NV + k] * Rn[(size_t)(k * NV + q) * W + cn];
+      __syncthreads();
+      A[c][s][e] = l;
+      Lb[c][e] = l;
+      __syncthreads();
+      if (nok) {
+        for (int ui = 0; ui < 6; ++ui) {
+          const int ts = kCUp[ui];
+          const int ri = di + kCDi[ts], rj = dj + kCDj[ts];
+          const int su = cslot(ri, rj);
+          if (su < 0 || !inside(c + ri, j + rj)) continue;
+          double sum = 0.0;
+          for (int k = 0; k < NV; ++k) sum += Lb[c][r * NV + k] * Rn[(size_t)(ts * B + k * NV + q) * W + cn];
+          A[c][su][e] -= sum;
+        }
+      }
+      __syncthreads();
+    }
+    if (e == 0 && valid) {
+      double D[B], Di[B];
+      for (int k = 0; k < B; ++k) D[k] = A[c][0][k];
+      cinv<NV>(D, Di);
+      for (int k = 0; k < B; ++k) Lb[c][k] = Di[k];
+    }
+    __syncthreads();
+    double* RL = R + (size_t)L * RS;
+    for (int s = 0; s < NSLOT; ++s)
+      RL[(size_t)(s * B + e) * W + c] = valid ? (s == 0 ? Lb[c][e] : A[c][s][e]) : 0.0;
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// Work-group barrier that orders LDS only: the level loop keeps the next
+// level's global loads in flight across it (the usual barrier also waits for
+// every outstanding global load).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Compile-time stencil tables for the apply kernel (fully unrolled loops
+// fold them away): offsets, the pass's neighbour slots, and the slot of the
+// opposite offset.
+namespace ct {
+constexpr int Di[NSLOT] = {0, 0, 0, -1, 1, -1, 1, -1, 1, 0, 0, -2, 2};
+constexpr int Dj[NSLOT] = {0, -1, 1, 0, 0, -1, -1, 1, 1, -2, 2, 0, 0};
+constexpr int Low[6] = {11, 5, 3, 7, 9, 1};
+constexpr int Up[6] = {2, 10, 6, 4, 8, 12};
+constexpr int Back[NSLOT] = {0, 2, 1, 4, 3, 8, 7, 6, 5, 10, 9, 12, 11};
+}  // namespace ct
+
+// One level's operands of thread (k, c): f[n*NV + m] for the six neighbours n
+// of the pass, d[m] for the diagonal inverse (row k, or column k when
+// transposed), x the right-hand side.  Every load is unconditional (clamped
+// to an in-range address) and masked afterwards, so the loads of a level can
+// all be in flight together.
+template <int NV, int W, bool TR, bool ASC>
+__device__ __forceinline__ void cilu_fetch(const double* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
+                                           int k, int c, int L, const double* src, double* f, double* d, double& x) {
+  constexpr int B = NV * NV, RS = NSLOT * B * W;
+  const int j = L - 2 * c;
+  const bool lok = L >= 0 && L < nlev;
+  const bool valid = lok && c < wk && j >= 0 && j < njc;
+  const int Lc = lok ? L : 0;
+  const double* RL = R + (size_t)Lc * RS;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    const int s = ASC ? ct::Low[n] : ct::Up[n];
+    const int cn = c + ct::Di[s], jn = j + ct::Dj[s];
+    const bool nok = valid && cn >= 0 && cn < wk && jn >= 0 && jn < njc;
+    if (!TR) {
+      const double* p = RL + (size_t)(s * B + k * NV) * W + c;
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        const double v = p[(size_t)m * W];
+        f[n * NV + m] = nok ? v : 0.0;
+      }
+    } else {
+      // entry of the neighbour's row that points back to this cell
+      const double* Rn = R + (size_t)(nok ? jn + 2 * cn : Lc) * RS;
+      const double* p = Rn + (size_t)(ct::Back[s] * B + k) * W + (nok ? cn : c);
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        const double v = p[(size_t)m * NV * W];
+        f[n * NV + m] = nok ? v : 0.0;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < NV; ++m) {
+    const double v = RL[(size_t)(TR ? (m * NV + k) : (k * NV + m)) * W + c];
+    d[m] = valid ? v : 0.0;
+  }
+  const double xv = src[k * plane + (valid ? (size_t)j * ldi + i0 + c : 0)];
+  x = valid ? xv : 0.0;
+}
+
+// One level of a triangular pass with its operands already in registers.
+template <int NV, int W, bool ASC, bool DIAG>
+__device__ __forceinline__ void cilu_level(int L, int njc, int ldi, size_t plane, int i0, int wk, int k, int c,
+                                           const double* f, const double* d, double x, double* z,
+                                           double (*ring)[8][NV], double (*tmp)[NV]) {
+  constexpr int NR = 8;
+  const int j = L - 2 * c;
+  const bool valid = c < wk && j >= 0 && j < njc;
+  double a0 = x, a1 = 0.0;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    const int s = ASC ? ct::Low[n] : ct::Up[n];
+    const int cn = c + ct::Di[s], jn = j + ct::Dj[s];
+    // out-of-chunk / out-of-grid neighbours carry f = 0; read a valid slot
+    const bool nok = cn >= 0 && cn < W && jn >= 0;
+    const double* zr = ring[nok ? cn : c][(nok ? jn : j) & (NR - 1)];
+    double t = 0.0;
+#pragma unroll
+    for (int m = 0; m < NV; ++m) t += f[n * NV + m] * zr[m];
+    if (n & 1) a1 += t;
+    else a0 -= t;
+  }
+  double acc = a0 - a1;
+  if (DIAG) {
+    if (valid) tmp[c][k] = acc;
+    lds_barrier();
+    double v = 0.0;
+#pragma unroll
+    for (int m = 0; m < NV; ++m) v += d[m] * tmp[c][m];
+    acc = v;
+  }
+  if (valid) {
+    ring[c][j & (NR - 1)][k] = acc;
+    z[k * plane + (size_t)j * ldi + i0 + c] = acc;
+  }
+  lds_barrier();
+}
+
+// One triangular pass over the chunk's levels (ascending or descending),
+// applying the diagonal inverse when DIAG.  Two operand sets alternate so that
+// the next level's loads stay in flight while the current level computes.
+template <int NV, int W, bool TR, bool ASC, bool DIAG>
+__device__ __forceinline__ void cilu_pass(const double* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
+                                          int k, int c, const double* src, double* z, double (*ring)[8][NV],
+                                          double (*tmp)[NV]) {
+  double fa[6 * NV], da[NV], xa, fb[6 * NV], db[NV], xb;
+  auto lev = [&](int step) { return ASC ? step : nlev - 1 - step; };
+  cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(0), src, fa, da, xa);
+  for (int step = 0; step < nlev; step += 2) {
+    cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + 1), src, fb, db, xb);
+    cilu_level<NV, W, ASC, DIAG>(lev(step), njc, ldi, plane, i0, wk, k, c, fa, da, xa, z, ring, tmp);
+    if (step + 1 >= nlev) break;
+    cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + 2), src, fa, da, xa);
+    cilu_level<NV, W, ASC, DIAG>(lev(step + 1), njc, ldi, plane, i0, wk, k, c, fb, db, xb, z, ring, tmp);
+  }
+}
+
+// z = (LU)^-1 b, or (LU)^-T b when TR.  Thread (k, c): component k of the
+// cell in column c.  Operands are loaded one level ahead of their use.
+template <int NV, int W, bool TR>
+__global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, int ldi, int nlev, const double* rec,
+                                                            const double* b, double* z) {
+  constexpr int B = NV * NV, RS = NSLOT * B * W;
+  __shared__ double ring[W][8][NV];
+  __shared__ double tmp[W][NV];
+  const int chunk = blockIdx.x, i0 = chunk * W, wk = min(W, nic - i0);
+  const size_t plane = (size_t)njc * ldi;
+  const int k = threadIdx.x / W, c = threadIdx.x % W;
+  const double* R = rec + (size_t)chunk * nlev * RS;
+  // ring cells never written hold zeros (their f is zero too)
+  for (int e = threadIdx.x; e < W * 8 * NV; e += NV * W) (&ring[0][0][0])[e] = 0.0;
+  __syncthreads();
+  // non-transposed: unit-lower L ascending, then D + U descending;
+  // transposed: (D + U)^T ascending, then unit-lower L^T descending
+  cilu_pass<NV, W, TR, true, TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, b, z, ring, tmp);
+  // the second pass reads what the first stored: complete every store first
+  __syncthreads();
+  cilu_pass<NV, W, TR, false, !TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, z, z, ring, tmp);
+}
+
+size_t cilu_size(int nic, int njc, int nv, int w) {
+  const int nchunk = (nic + w - 1) / w, nlev = njc + 2 * w - 2;
+  return (size_t)nchunk * nlev * NSLOT * nv * nv * w;
+}
+
+template <int NV, int W>
+static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const double* blocks, double* rec,
+                          hipStream_t s) {
+  const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2;
+  hipLaunchKernelGGL((cilu_factor_kernel<NV, W>), dim3(nchunk), dim3(W * NV * NV), 0, s, nic, njc, ldi, nlev, idt,
+                     blocks, rec);
+}
+
+template <int NV, int W>
+static void cilu_apply_t(int nic, int njc, int ldi, const double* rec, const double* b, double* z, bool tr,
+                         hipStream_t s) {
+  const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2;
+  if (tr)
+    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, true>), dim3(nchunk), dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec, b,
+                       z);
+  else
+    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, false>), dim3(nchunk), dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec,
+                       b, z);
+}
+
+bool cilu_width_ok(int w) { return w == 8 || w == 16 || w == 32; }
+
+hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, const double* idt, const double* blocks, double* rec,
+                       hipStream_t s) {
+  if (nv == 5) {
+    if (w == 8) cilu_factor_t<5, 8>(nic, njc, ldi, idt, blocks, rec, s);
+    else if (w == 16) cilu_factor_t<5, 16>(nic, njc, ldi, idt, blocks, rec, s);
+    else cilu_factor_t<5, 32>(nic, njc, ldi, idt, blocks, rec, s);
+  } else {
+    if (w == 8) cilu_factor_t<4, 8>(nic, njc, ldi, idt, blocks, rec, s);
+    else if (w == 16) cilu_factor_t<4, 16>(nic, njc, ldi, idt, blocks, rec, s);
+    else cilu_factor_t<4, 32>(nic, njc, ldi, idt, blocks, rec, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t cilu_apply(int nic, int njc, int ldi, int nv, int w, const double* rec, const double* b, double* z,
+                      bool transpose, hipStream_t s) {
+  if (nv == 5) {
+    if (w == 8) cilu_apply_t<5, 8>(nic, njc, ldi, rec, b, z, transpose, s);
+    else if (w == 16) cilu_apply_t<5, 16>(nic, njc, ldi, rec, b, z, transpose, s);
+    else cilu_apply_t<5, 32>(nic, njc, ldi, rec, b, z, transpose, s);
+  } else {
+    if (w == 8) cilu_apply_t<4, 8>(nic, njc, ldi, rec, b, z, transpose, s);
+    else if (w == 16) cilu_apply_t<4, 16>(nic, njc, ldi, rec, b, z, transpose, s);
+    else cilu_apply_t<4, 32>(nic, njc, ldi, rec, b, z, transpose, s);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace sg

@@ -41,9 +41,12 @@ struct SolverState {
   int m = 0;                 // restart length
   size_t n = 0;              // vector length (nvar * plane)
   DevBuf V, Z, blocks, fac, ilu, idt, dtv, rhs, x, tmp, jz, qop, seed, hdev, ydev, partial, gst, dbdiag, ones;
+  DevBuf blocks_pc;          // preconditioner blocks when lhs_order > 0 (blocks keeps the exact J)
+  DevBuf cilu;               // chunked line ILU(0) factors (precond 6)
+  int pc_width = 16;         // chunk width of precond 6
   bool have_lhs = false, have_rhs = false;
   int precond = 5;           // 0 none, 1 block Jacobi, 2 j-line, 3 line Gauss-Seidel, 4 multicolour ILU(0),
-                             // 5 level-scheduled natural-order ILU(0)
+                             // 5 level-scheduled natural-order ILU(0), 6 chunked line ILU(0)
   int sweeps = 1;
   double pc_boost = 1.0;     // preconditioner uses idt * pc_boost
   int lhs_order = 0;         // > 0: preconditioner blocks from the residual at this MUSCL order
@@ -104,6 +107,10 @@ int precond_apply(sg_ctx* ctx, const double* v, double* z) {
     SG_TRY(lilu_apply(nic, njc, ldi, nv, S->ilu.p, v, z, S->transpose, ctx->stream), "ilu apply");
     return SG_OK;
   }
+  if (S->precond == 6) {
+    SG_TRY(cilu_apply(nic, njc, ldi, nv, S->pc_width, S->cilu.p, v, z, S->transpose, ctx->stream), "cilu apply");
+    return SG_OK;
+  }
   if (S->precond <= 2 && !S->transpose) {
     SG_TRY(line_solve(nic, njc, ldi, nv, S->fac.p, v, z, ctx->stream), "precond");
     return SG_OK;
@@ -142,9 +149,10 @@ int apply_operator(sg_ctx* ctx, const double* z, double* w) {
 // Blocks of J at qop by 5 x 5 colouring of J.v.  order > 0 evaluates the
 // residual at that MUSCL order (the reference's lhs_order: a first-order
 // Jacobian for the implicit operator, solver.cpp:92-101).
-int extract_jacobian(sg_ctx* ctx, int order = 0) {
+int extract_jacobian(sg_ctx* ctx, int order = 0, double* dst = nullptr) {
   sg::DevPhys ph = ctx->ph;
   if (order > 0) ph.order = order;
+  if (!dst) dst = ctx->solver->blocks.p;
   SolverState* S = ctx->solver;
   const int nic = ctx->geo.nic, njc = ctx->geo.njc, ldi = ctx->g.ldi, nv = ctx->nvar;
   for (int a = 0; a < 5; ++a)
@@ -153,9 +161,16 @@ int extract_jacobian(sg_ctx* ctx, int order = 0) {
         SG_TRY(seed_colour(nic, njc, ldi, nv, a, b, m, S->seed.p, ctx->stream), "seed");
         SG_TRY(launch_jv(ctx->g, ph, ctx->bc, ctx->sa, S->qop.p, S->seed.p, nullptr, S->jz.p, ctx->stream, nullptr,
                          ctx->clo), "jv");
-        SG_TRY(extract_blocks(nic, njc, ldi, nv, a, b, m, S->jz.p, S->blocks.p, ctx->stream), "extract");
+        SG_TRY(extract_blocks(nic, njc, ldi, nv, a, b, m, S->jz.p, dst, ctx->stream), "extract");
       }
   return SG_OK;
+}
+
+// Blocks the preconditioner factors: the first-order Jacobian when lhs_order
+// > 0, else the exact one.
+const double* pc_blocks(sg_ctx* ctx) {
+  SolverState* S = ctx->solver;
+  return (S->lhs_order > 0 && S->blocks_pc.p) ? S->blocks_pc.p : S->blocks.p;
 }
 
 int factor_preconditioner(sg_ctx* ctx, const double* idt_pc) {
@@ -163,27 +178,45 @@ int factor_preconditioner(sg_ctx* ctx, const double* idt_pc) {
   if (S->precond == 0) return SG_OK;
   const int nic = ctx->geo.nic, njc = ctx->geo.njc, ldi = ctx->g.ldi, nv = ctx->nvar;
   const double* pc_idt = idt_pc;
+  const double* blk = pc_blocks(ctx);
   if (S->pc_boost > 1.0) {  // boosted diagonal for the factorisation only
     SG_TRY(axpby(S->pc_boost, idt_pc, 0.0, S->gst.p, ctx->g.plane, ctx->stream), "boost");
     pc_idt = S->gst.p;
   }
   if (S->precond == 4) {
-    SG_TRY(ilu_factor(nic, njc, ldi, nv, pc_idt, S->blocks.p, S->ilu.p, ctx->stream), "ilu factor");
+    SG_TRY(ilu_factor(nic, njc, ldi, nv, pc_idt, blk, S->ilu.p, ctx->stream), "ilu factor");
     return SG_OK;
   }
   if (S->precond == 5) {
-    SG_TRY(lilu_factor(nic, njc, ldi, nv, pc_idt, S->blocks.p, S->ilu.p, ctx->stream), "ilu factor");
+    SG_TRY(lilu_factor(nic, njc, ldi, nv, pc_idt, blk, S->ilu.p, ctx->stream), "ilu factor");
     return SG_OK;
   }
-  SG_TRY(line_factor(nic, njc, ldi, nv, pc_idt, S->blocks.p, S->fac.p, S->precond == 1, ctx->stream), "factor");
+  if (S->precond == 6) {
+    const size_t need = cilu_size(nic, njc, nv, S->pc_width);
+    if (S->cilu.n < need && S->cilu.alloc(need) != hipSuccess) return sg_fail(ctx, SG_ERR_HIP, "allocation failed");
+    SG_TRY(cilu_factor(nic, njc, ldi, nv, S->pc_width, pc_idt, blk, S->cilu.p, ctx->stream), "cilu factor");
+    return SG_OK;
+  }
+  SG_TRY(line_factor(nic, njc, ldi, nv, pc_idt, blk, S->fac.p, S->precond == 1, ctx->stream), "factor");
   return SG_OK;
+}
+
+// Blocks of the preconditioner at qop (the exact blocks for lhs_order 0).
+int extract_pc_blocks(sg_ctx* ctx) {
+  SolverState* S = ctx->solver;
+  if (S->lhs_order > 0) {
+    if (!S->blocks_pc.p && S->blocks_pc.alloc(S->blocks.n) != hipSuccess)
+      return sg_fail(ctx, SG_ERR_HIP, "allocation failed");
+    return extract_jacobian(ctx, S->lhs_order, S->blocks_pc.p);
+  }
+  return extract_jacobian(ctx);
 }
 
 int build_preconditioner(sg_ctx* ctx) {
   SolverState* S = ctx->solver;
   if (S->precond == 0) return SG_OK;
   int rc;
-  if ((rc = extract_jacobian(ctx, S->lhs_order))) return rc;
+  if ((rc = extract_pc_blocks(ctx))) return rc;
   return factor_preconditioner(ctx, S->idt.p);
 }
 
@@ -300,13 +333,22 @@ extern "C" {
 
 // ---- LinearSolverGMRES: the reference's linear-solver interface ----------
 int sg_ls_preallocate(sg_ctx* ctx, int restart, int precond, int maxit, double rtol) {
-  if (!ctx || restart < 1 || restart > 200 || precond < 0 || precond > 5) return SG_ERR_ARG;
+  if (!ctx || restart < 1 || restart > 200 || precond < 0 || precond > 6) return SG_ERR_ARG;
   int rc = ensure(ctx, restart);
   if (rc) return rc;
   ctx->solver->precond = precond;
   ctx->solver->pc_boost = 1.0;
   ctx->solver->maxit = maxit > 0 ? maxit : restart;
   ctx->solver->rtol = rtol > 0 ? rtol : 1e-3;
+  return SG_OK;
+}
+
+int sg_ls_set_pc(sg_ctx* ctx, int lhs_order, double pc_dt_factor, int pc_width) {
+  if (!ctx || !ctx->solver || lhs_order < 0 || lhs_order > 2 || !(pc_dt_factor >= 1.0)) return SG_ERR_ARG;
+  if (pc_width != 0 && !cilu_width_ok(pc_width)) return sg_fail(ctx, SG_ERR_ARG, "pc_width must be 8, 16 or 32");
+  ctx->solver->lhs_order = lhs_order;
+  ctx->solver->pc_boost = pc_dt_factor;
+  if (pc_width) ctx->solver->pc_width = pc_width;
   return SG_OK;
 }
 
@@ -426,6 +468,7 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     Timer tj;
     S->pc_boost = (o->pc_cfl_max > 0.0 && cfl > o->pc_cfl_max) ? cfl / o->pc_cfl_max : 1.0;
     S->lhs_order = o->lhs_order;
+    if (o->pc_width) S->pc_width = o->pc_width;
     SG_TRY(hipMemcpyAsync(S->qop.p, q, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
     if ((rc = build_preconditioner(ctx))) return rc;
     SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
@@ -488,13 +531,16 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
   return converged ? SG_OK : SG_ERR_NOCONV;
 }
 
-// z = M^-1 v with the preconditioner of the last set_lhs (diagnostic hook).
-int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos) {
+// z = M^-1 v or M^-T v with the preconditioner of the last set_lhs.
+int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos, int transpose) {
   if (!ctx || !v_aos || !z_aos || !ctx->solver || !ctx->solver->have_lhs) return SG_ERR_STATE;
   SolverState* S = ctx->solver;
   int rc;
   if ((rc = upload(ctx, v_aos, S->tmp.p))) return rc;
-  if ((rc = precond_apply(ctx, S->tmp.p, S->x.p))) return rc;
+  S->transpose = transpose != 0;
+  rc = precond_apply(ctx, S->tmp.p, S->x.p);
+  S->transpose = false;
+  if (rc) return rc;
   return download(ctx, S->x.p, z_aos);
 }
 
@@ -570,7 +616,9 @@ int sg_jtv(sg_ctx* ctx, const double* q_aos, const double* psi_aos, double* jtps
 int sg_adjoint_solve(sg_ctx* ctx, const double* q_aos, const double* rhs_aos, double* psi_aos,
                      const sg_adjoint_opts* o, sg_adjoint_report* rep) {
   if (!ctx || !q_aos || !rhs_aos || !psi_aos || !o) return SG_ERR_ARG;
-  if (o->precond != 0 && o->precond != 4 && o->precond != 5) return sg_fail(ctx, SG_ERR_ARG, "adjoint precond must be 0, 4 or 5");
+  if (o->precond != 0 && o->precond != 4 && o->precond != 5 && o->precond != 6)
+    return sg_fail(ctx, SG_ERR_ARG, "adjoint precond must be 0, 4, 5 or 6");
+  if (o->lhs_order < 0 || o->lhs_order > 2) return sg_fail(ctx, SG_ERR_ARG, "adjoint lhs_order must be 0, 1 or 2");
   int rc;
   if ((rc = sg_ls_preallocate(ctx, o->restart, o->precond, o->maxit, o->rtol))) return rc;
   SolverState* S = ctx->solver;
@@ -581,7 +629,15 @@ int sg_adjoint_solve(sg_ctx* ctx, const double* q_aos, const double* rhs_aos, do
   SG_TRY(launch_dt(ctx->g, ctx->ph.mu_inf, ctx->nvar, ctx->q.p, ctx->dsmin.p, o->pc_cfl, S->dtv.p, ctx->stream), "dt");
   SG_TRY(invert_dt(S->dtv.p, S->idt.p, ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->stream), "invert");
   S->pc_boost = 1.0;
-  if ((rc = factor_preconditioner(ctx, S->idt.p))) return rc;
+  S->lhs_order = o->lhs_order;
+  if (S->lhs_order > 0) {
+    if (!S->blocks_pc.p && S->blocks_pc.alloc(S->blocks.n) != hipSuccess)
+      return sg_fail(ctx, SG_ERR_HIP, "allocation failed");
+    if ((rc = extract_jacobian(ctx, S->lhs_order, S->blocks_pc.p))) return rc;
+  }
+  rc = factor_preconditioner(ctx, S->idt.p);
+  S->lhs_order = 0;
+  if (rc) return rc;
   SG_TRY(hipMemsetAsync(S->idt.p, 0, ctx->g.plane * sizeof(double), ctx->stream), "memset");  // steady operator
   if ((rc = upload(ctx, rhs_aos, S->rhs.p))) return rc;
   SG_TRY(axpby(0.0, S->rhs.p, -1.0, S->rhs.p, S->n, ctx->stream), "negate");  // (-J)^T psi = -rhs

@@ -18,22 +18,23 @@ import numpy as np
 
 from . import cases, lib
 
-# Newton-Krylov settings of the SA solves (pseudo-transient continuation with
-# switched evolution relaxation on the flux-balance norm).
-NEWTON = dict(cfl=5.0, ser=1, ser_exponent=1.5, cfl_max=1e10, gmres_rtol=1e-4, precond=4, pc_cfl_max=300.0,
-              gmres_restart=40, gmres_maxit=200, iteration_max=400, norm_weighted=1, tolerance=1e-12)
-# warm starts (a converged state of a nearby problem): the residual first rises
-# while the boundary layer adapts, so the pseudo-time step starts larger; much
-# larger starting CFL numbers outrun the preconditioner (its factorisation is
-# capped at pc_cfl_max) and stall GMRES
-WARM = dict(NEWTON, cfl=100.0)
+# Newton-Krylov settings of the SA solves: pseudo-transient continuation with
+# switched evolution relaxation on the flux-balance norm; GMRES(30)
+# preconditioned by the chunked line ILU(0) of the first-order Jacobian
+# (lhs_order 1), which stays effective up to CFL 1e10 on 1e-6 wall spacing.
+NEWTON = dict(cfl=5.0, ser=1, ser_exponent=1.5, cfl_max=1e10, gmres_rtol=1e-4, precond=6, lhs_order=1,
+              pc_width=16, pc_cfl_max=0.0, gmres_restart=30, gmres_maxit=100, iteration_max=200, norm_weighted=1,
+              tolerance=1e-12)
+# warm starts (a converged state of a nearby problem): the initial residual is
+# small, so SER would keep the CFL near its start; start high instead
+WARM = dict(NEWTON, cfl=1e5)
 RESOLVE = dict(WARM, iteration_max=150)
 # the BF16 closure network makes beta a piecewise-constant function of the
 # state at its rounding points, which sets a floor on the flux-balance norm
 # (measured ~1.5e-11 on the 65x49 plate against 2e-12 for plain SA)
 MLP_TOL = 1e-10
 MLP_STALL = 8
-ADJOINT = dict(restart=60, maxit=3000, rtol=1e-8, precond=4, pc_cfl=300.0)
+ADJOINT = dict(restart=60, maxit=3000, rtol=1e-8, precond=6, lhs_order=1, pc_cfl=1e4)
 
 
 def truth_coef(seed: int = 0, amp: float = 0.3) -> np.ndarray:

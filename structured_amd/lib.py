@@ -212,6 +212,9 @@ class Equation:
     def ls_preallocate(self, restart=30, precond=2, maxit=60, rtol=1e-3):
         self._check(load().sg_ls_preallocate(self.h, restart, precond, maxit, C.c_double(rtol)))
 
+    def ls_set_pc(self, lhs_order=0, pc_dt_factor=1.0, pc_width=0):
+        self._check(load().sg_ls_set_pc(self.h, int(lhs_order), C.c_double(pc_dt_factor), int(pc_width)))
+
     def ls_set_lhs(self, q, dt):
         self._check(load().sg_ls_set_lhs(self.h, _p(_f64(q, self.shape)), _p(_f64(dt, self.shape))))
 
@@ -227,17 +230,18 @@ class Equation:
             self._check(rc)
         return dq, it.value, rr.value
 
-    def ls_precond_apply(self, v):
+    def ls_precond_apply(self, v, transpose=False):
         z = np.empty(self.shape)
-        self._check(load().sg_ls_precond_apply(self.h, _p(_f64(v, self.shape)), _p(z)))
+        self._check(load().sg_ls_precond_apply(self.h, _p(_f64(v, self.shape)), _p(z), int(bool(transpose))))
         return z
 
     # ---- adjoint
-    def adjoint_solve(self, q, rhs, restart=40, maxit=400, rtol=1e-10, precond=4, pc_cfl=1e3):
+    def adjoint_solve(self, q, rhs, restart=40, maxit=400, rtol=1e-10, precond=4, pc_cfl=1e3, lhs_order=0):
         q = _f64(q, self.shape)
         rhs = _f64(rhs, self.shape)
         psi = np.empty(self.shape)
-        o = abi.AdjointOpts(restart=restart, maxit=maxit, rtol=rtol, precond=precond, reserved=0, pc_cfl=pc_cfl)
+        o = abi.AdjointOpts(restart=restart, maxit=maxit, rtol=rtol, precond=precond, lhs_order=lhs_order,
+                            pc_cfl=pc_cfl)
         rep = abi.AdjointReport()
         rc = load().sg_adjoint_solve(self.h, _p(q), _p(rhs), _p(psi), C.byref(o), C.byref(rep))
         if rc not in (0, -5):
@@ -252,12 +256,13 @@ class Equation:
         self._check(load().sg_objective_cf(self.h, _p(q), _p(t), C.byref(F), _p(g)))
         return F.value, g
 
-    def adjoint_gradient(self, q, cf_target, restart=40, maxit=400, rtol=1e-10, precond=4, pc_cfl=1e3):
+    def adjoint_gradient(self, q, cf_target, restart=40, maxit=400, rtol=1e-10, precond=4, pc_cfl=1e3, lhs_order=0):
         q = _f64(q, self.shape)
         t = _f64(cf_target)
         gb = np.empty((self.nic, self.njc))
         F = C.c_double()
-        o = abi.AdjointOpts(restart=restart, maxit=maxit, rtol=rtol, precond=precond, reserved=0, pc_cfl=pc_cfl)
+        o = abi.AdjointOpts(restart=restart, maxit=maxit, rtol=rtol, precond=precond, lhs_order=lhs_order,
+                            pc_cfl=pc_cfl)
         rep = abi.AdjointReport()
         rc = load().sg_adjoint_gradient(self.h, _p(q), _p(t), _p(gb), C.byref(F), C.byref(o), C.byref(rep))
         if rc not in (0, -5):
@@ -272,7 +277,7 @@ class Equation:
         J history)."""
         q = _f64(q, self.shape).copy()
         b = np.ones((self.nic, self.njc)) if beta is None else _f64(beta, (self.nic, self.njc)).copy()
-        adj = dict(restart=60, maxit=2000, rtol=1e-10, precond=4, reserved=0, pc_cfl=300.0)
+        adj = dict(restart=60, maxit=2000, rtol=1e-10, precond=4, lhs_order=0, pc_cfl=300.0)
         adj.update(adjoint or {})
         o = abi.InversionOpts(iterations=iterations, memory=memory, reg=reg, step0=step0, beta_min=beta_min,
                               beta_max=beta_max, max_backtracks=max_backtracks, reserved=0,

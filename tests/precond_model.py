@@ -1,6 +1,7 @@
 """numpy restatement of the library's Jacobian-block preconditioners (test
-infrastructure): block Jacobi, j-line block-tridiagonal, and symmetric
-multicolour j-line Gauss-Seidel, built from a dense Jacobian."""
+infrastructure): block Jacobi, j-line block-tridiagonal, symmetric
+multicolour j-line Gauss-Seidel, five-colour, natural-order and chunked line
+block ILU(0), built from a dense Jacobian."""
 import numpy as np
 
 SLOTS = [(0, 0), (0, -1), (0, 1), (-1, 0), (1, 0), (-1, -1), (1, -1), (-1, 1), (1, 1), (0, -2), (0, 2), (-2, 0), (2, 0)]
@@ -33,11 +34,13 @@ def line_solve(B, idt, i, t, jacobi=False):
     return np.linalg.solve(A, t.reshape(-1)).reshape(njc, nv)
 
 
-def apply(B, idt, v, precond, sweeps=1):
+def apply(B, idt, v, precond, sweeps=1, width=16):
     v = np.asarray(v, dtype=np.float64)
     nic, njc, nv = v.shape
     if precond == 0:
         return v.copy()
+    if precond == 6:
+        return ilu_chunked(B, idt, v, width)
     if precond == 4:
         return ilu_mc5(B, idt, v)
     if precond == 5:
@@ -126,18 +129,18 @@ def ilu_mc5(B, idt, v):
     return z
 
 
-def ilu_natural(B, idt, v, transpose=False):
-    """Block ILU(0) of I/dt - J in natural j-major order (o = j*nic + i)."""
+def _ilu_ordered(B, idt, v, order, keep, transpose=False):
+    """Block ILU(0) of I/dt - J on the 13-point stencil with the cells in the
+    given order, keeping only the couplings keep((i,j), (ii,jj)) accepts."""
     v = np.asarray(v, dtype=np.float64)
     nic, njc, nv = v.shape
-    order = [(i, j) for j in range(njc) for i in range(nic)]
     pos = {c: n for n, c in enumerate(order)}
     rows = []
     for (i, j) in order:
         d = {}
         for s, (di, dj) in enumerate(SLOTS):
             ii, jj = i + di, j + dj
-            if 0 <= ii < nic and 0 <= jj < njc:
+            if 0 <= ii < nic and 0 <= jj < njc and keep((i, j), (ii, jj)):
                 blk = -B[i, j, s].copy()
                 if s == 0:
                     blk += np.eye(nv) * idt[i, j]
@@ -186,3 +189,18 @@ def ilu_natural(B, idt, v, transpose=False):
     for n, c in enumerate(order):
         out[c] = x[n]
     return out
+
+
+def ilu_natural(B, idt, v, transpose=False):
+    """Block ILU(0) of I/dt - J in natural j-major order (o = j*nic + i)."""
+    nic, njc = np.asarray(v).shape[:2]
+    order = [(i, j) for j in range(njc) for i in range(nic)]
+    return _ilu_ordered(B, idt, v, order, lambda a, b: True, transpose)
+
+
+def ilu_chunked(B, idt, v, w, transpose=False):
+    """Chunked line block ILU(0): i-chunks of w columns, each ordered column
+    by column with j fastest; couplings between chunks dropped."""
+    nic, njc = np.asarray(v).shape[:2]
+    order = [(i, j) for i in range(nic) for j in range(njc)]  # chunks are contiguous in i
+    return _ilu_ordered(B, idt, v, order, lambda a, b: a[0] // w == b[0] // w, transpose)

@@ -111,3 +111,53 @@ def test_ilu_matches_numpy_model_sa(precond):
     err = np.abs(z - zr).max() / np.abs(zr).max()
     print("SA precond", precond, "model err", err)
     assert err <= 1e-9
+
+
+@pytest.mark.parametrize("width,sa", [(8, False), (16, False), (8, True)])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_chunked_ilu_matches_numpy_model(width, sa, transpose):
+    """Chunked line ILU(0) (precond 6): chunks of `width` columns (a partial
+    last chunk at width 8 on 12 columns), M^-1 and M^-T against the model."""
+    import precond_model as pm
+    case = cases.sa_bump(13, 9, dy1=1e-3) if sa else cases.laminar_plate(13, 9, dy1=1e-2)
+    q = case.perturbed_state(seed=6, amp=0.01)
+    eq = lib.Equation(case)
+    dt = eq.calc_dt(q, 50.0)
+    eq.ls_preallocate(restart=10, precond=6, maxit=10, rtol=1e-6)
+    eq.ls_set_pc(0, 1.0, width)
+    eq.ls_set_lhs(q, dt)
+    v = np.random.default_rng(3).standard_normal(q.shape)
+    z = eq.ls_precond_apply(v, transpose=transpose)
+    J = of.jacobian(case, q, wd=eq.walldist() if sa else None)
+    B = pm.blocks_from_dense(J, case.nic, case.njc, case.nvar)
+    zr = pm.ilu_chunked(B, 1.0 / dt[..., 0], v, width, transpose=transpose)
+    err = float(np.abs(z - zr).max() / np.abs(zr).max())
+    print("chunked ILU width", width, "sa", sa, "transpose", transpose, "model err", err)
+    assert err <= 1e-9
+
+
+def test_first_order_preconditioner_blocks():
+    """lhs_order = 1: the preconditioner factors the Jacobian of the
+    first-order residual (the oracle's J at order 1), not the exact one."""
+    import copy
+    import precond_model as pm
+    case = cases.laminar_plate(13, 9, dy1=1e-2)
+    q = case.perturbed_state(seed=7, amp=0.01)
+    eq = lib.Equation(case)
+    dt = eq.calc_dt(q, 50.0)
+    eq.ls_preallocate(restart=10, precond=5, maxit=10, rtol=1e-6)
+    eq.ls_set_pc(1, 1.0, 0)
+    eq.ls_set_lhs(q, dt)
+    v = np.random.default_rng(4).standard_normal(q.shape)
+    z = eq.ls_precond_apply(v)
+    c1 = copy.deepcopy(case)
+    c1.order = 1
+    B1 = pm.blocks_from_dense(of.jacobian(c1, q), case.nic, case.njc, case.nvar)
+    B2 = pm.blocks_from_dense(of.jacobian(case, q), case.nic, case.njc, case.nvar)
+    z1 = pm.ilu_natural(B1, 1.0 / dt[..., 0], v)
+    z2 = pm.ilu_natural(B2, 1.0 / dt[..., 0], v)
+    err = float(np.abs(z - z1).max() / np.abs(z1).max())
+    print("first-order preconditioner model err", err, "distance to second order",
+          float(np.abs(z2 - z1).max() / np.abs(z1).max()))
+    assert err <= 1e-9
+    assert np.abs(z2 - z1).max() > 1e-3 * np.abs(z1).max()
