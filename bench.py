@@ -24,6 +24,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -41,7 +43,85 @@ def parse():
     ap.add_argument("--min-bytes", type=float, default=600e6, help="rotating working set")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
+    ap.add_argument("--no-closure", action="store_true", help="skip the ML-closure vs plain-SA leg")
+    ap.add_argument("--adjoint-grid", default="705x321")
+    ap.add_argument("--closure-grid", default="1409x641")
     return ap.parse_args()
+
+
+def _grid(s):
+    ni, nj = s.lower().split("x")
+    return int(ni), int(nj)
+
+
+def adjoint_leg(grid):
+    """Adjoint beta-gradient of the wall-Cf objective on the channel bump:
+    beta = 1 + 8 seeded blobs, target Cf from a second seeded beta*.  The
+    primal states are converged first (untimed for the metric); the metric is
+    the wall time of sg_adjoint_gradient (objective, dF/dq, Jacobian blocks,
+    preconditioner, transposed GMRES, dR/dbeta)."""
+    from structured_amd import cases, fiml, lib
+    ni, nj = _grid(grid)
+    case = cases.sa_bump(ni, nj)
+    eq = lib.Equation(case)
+    t0 = time.perf_counter()
+    q1, r1, _ = eq.solve(case.freestream_state(), **fiml.NEWTON)
+    t_cold = time.perf_counter() - t0
+    eq.set_beta(cases.blob_beta(case, seed=2))
+    qs, r2, _ = eq.solve(q1, **fiml.WARM)
+    _, _, cf_target, _ = eq.wall(qs)
+    eq.set_beta(cases.blob_beta(case, seed=1))
+    t0 = time.perf_counter()
+    q, r3, _ = eq.solve(q1, **fiml.WARM)
+    t_warm = time.perf_counter() - t0
+    adj = dict(fiml.ADJOINT, restart=120, maxit=4000)
+    t0 = time.perf_counter()
+    g, F, rep = eq.adjoint_gradient(q, cf_target, **adj)
+    t_adj = time.perf_counter() - t0
+    out = {"grid": grid, "cells": case.nic * case.njc, "adjoint_gradient_s": t_adj,
+           "gmres_iterations": rep["iterations"], "relres": rep["relres"], "rtol": adj["rtol"],
+           "converged": rep["converged"], "F": F, "grad_max": float(np.abs(g).max()),
+           "primal_cold_solve_s": t_cold, "primal_cold_newton": r1["iterations"],
+           "primal_warm_solve_s": t_warm, "primal_converged": [bool(r["converged"]) for r in (r1, r2, r3)],
+           "inversion_iteration_s_estimate": t_warm + t_adj}
+    eq.close()
+    return out
+
+
+def closure_leg(grid):
+    """Plain-SA solve against the solve with the closure network fused into
+    the SA source (BF16 MFMA inference in every residual and J.v), both from
+    freestream on the same bump.  The network is trained on the GPU first
+    (untimed for the metric) on the features of the plain-SA solution against
+    a seeded analytic beta*."""
+    from structured_amd import cases, fiml, lib
+    ni, nj = _grid(grid)
+    case = cases.sa_bump(ni, nj, height=0.065, width=0.25, re=5e6)
+    eq = lib.Equation(case)
+    t0 = time.perf_counter()
+    q_sa, r_sa, _ = eq.solve(case.freestream_state(), **fiml.NEWTON)
+    t_sa = time.perf_counter() - t0
+    coef = fiml.truth_coef(seed=1)
+    eq.set_closure(1, coef=coef)
+    feats, beta_t = eq.closure_eval(q_sa)
+    net, hist, t_train = fiml.train(feats.reshape(-1, 5).astype(np.float32),
+                                    (beta_t - 1.0).reshape(-1).astype(np.float32), epochs=20, batch=65536, lr=1e-3)
+    eq.set_closure_mlp(net)
+    o = dict(fiml.NEWTON, tolerance=fiml.MLP_TOL, stall_window=fiml.MLP_STALL)
+    t0 = time.perf_counter()
+    q_ml, r_ml, _ = eq.solve(case.freestream_state(), **o)
+    t_ml = time.perf_counter() - t0
+    _, _, cf_sa, _ = eq.wall(q_sa)
+    _, _, cf_ml, _ = eq.wall(q_ml)
+    out = {"grid": grid, "cells": case.nic * case.njc, "plain_sa_solve_s": t_sa, "plain_sa_newton": r_sa["iterations"],
+           "plain_sa_converged": bool(r_sa["converged"]), "ml_closure_solve_s": t_ml,
+           "ml_closure_newton": r_ml["iterations"], "ml_closure_converged": bool(fiml.closure_solve_ok(r_ml)),
+           "ml_over_sa_time": t_ml / t_sa, "ml_l2w_final": max(r_ml["l2w_final"]),
+           "cf_change_vs_sa": fiml.cf_error(cf_ml, cf_sa), "train_s": t_train, "train_loss": [float(hist[0]),
+                                                                                               float(hist[-1])]}
+    eq.close()
+    return out
 
 
 def cpu_baseline(case, q, budget_s):
@@ -162,6 +242,11 @@ def main():
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, q0, a.cpu_seconds)
+    eq.close()
+    if not a.no_adjoint:
+        out["adjoint_gradient"] = adjoint_leg(a.adjoint_grid)
+    if not a.no_closure:
+        out["ml_closure_vs_sa"] = closure_leg(a.closure_grid)
     print(json.dumps(out))
 
 

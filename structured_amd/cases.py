@@ -140,6 +140,20 @@ def sa_bump(ni: int = 705, nj: int = 321, height: float = 0.05, width: float = 0
                 tail=1, meta=dict(height=height, width=width))
 
 
+def blob_beta(case: Case, seed: int, nblob: int = 8, amp: float = 0.2) -> np.ndarray:
+    """Per-cell beta = 1 + the sum of nblob seeded Gaussian blobs of
+    amplitude +-amp and radius 0.05-0.2, centred near the lower wall."""
+    rng = np.random.default_rng(seed)
+    xc = 0.25 * (case.xv[:-1, :-1] + case.xv[1:, :-1] + case.xv[:-1, 1:] + case.xv[1:, 1:])
+    yc = 0.25 * (case.yv[:-1, :-1] + case.yv[1:, :-1] + case.yv[:-1, 1:] + case.yv[1:, 1:])
+    b = np.ones_like(xc)
+    for _ in range(nblob):
+        cx, cy = rng.uniform(-1.0, 1.0), rng.uniform(0.0, 0.1)
+        r = rng.uniform(0.05, 0.2)
+        b += rng.choice([-amp, amp]) * np.exp(-((xc - cx) ** 2 + (yc - cy) ** 2) / r ** 2)
+    return b
+
+
 def channel_example(path_grid: str | None = None) -> Case:
     """The reference's laminar channel example (examples/laminar/channel,
     3x101 grid, isothermal walls, periodic in x, body force dpdx)."""

@@ -452,7 +452,7 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
       if (it == 0 || ++recoveries > 12) { rc = sg_fail(ctx, SG_ERR_NAN, "non-finite residual in the Newton loop"); break; }
       SG_TRY(hipMemcpyAsync(q, S->qop.p, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "restore");
       cfl_factor *= 0.1;
-      cfl = std::max(o->cfl, cfl * 0.1);
+      cfl = std::max(1e-3 * o->cfl, cfl * 0.1);
       continue;
     }
     if (lmax < o->tolerance) { converged = true; break; }
@@ -493,6 +493,9 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     }
     if (alpha < 1.0) cfl_factor = std::max(1e-6, cfl_factor * 0.5);
     else cfl_factor = std::min(1.0, cfl_factor * 2.0);
+    // a linear solve that made no headway says the pseudo-time step outran
+    // the preconditioner: back off hard
+    if (rr > 0.5) cfl_factor = std::max(1e-6, cfl_factor * 0.1);
     last_alpha = alpha;
     SG_TRY(axpby(ur * alpha, S->x.p, 1.0, q, n, ctx->stream), "update");
     t_lin += tl.s();
@@ -505,7 +508,9 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
       tot = std::sqrt(tot);
       ser_ref = std::max(ser_ref, tot);
       cfl = std::min(o->cfl * std::pow(ser_ref / std::max(tot, 1e-300), o->ser_exponent), o->cfl_max);
-      cfl = std::max(cfl * cfl_factor, o->cfl);
+      // limited updates shrink the step below the starting CFL as well (a
+      // warm start at a large CFL must be able to back off)
+      cfl = std::max(cfl * cfl_factor, 1e-3 * o->cfl);
     }
     if (o->under_relaxation_ramp && counter > o->under_relaxation_ramp_iteration)
       ur = std::min(std::pow(ur, o->under_relaxation_ramp_exponent), 10.0);

@@ -23,7 +23,7 @@ from . import cases, lib
 # preconditioned by the chunked line ILU(0) of the first-order Jacobian
 # (lhs_order 1), which stays effective up to CFL 1e10 on 1e-6 wall spacing.
 NEWTON = dict(cfl=5.0, ser=1, ser_exponent=1.5, cfl_max=1e10, gmres_rtol=1e-4, precond=6, lhs_order=1,
-              pc_width=16, pc_cfl_max=0.0, gmres_restart=30, gmres_maxit=100, iteration_max=200, norm_weighted=1,
+              pc_width=32, pc_cfl_max=0.0, gmres_restart=30, gmres_maxit=150, iteration_max=200, norm_weighted=1,
               tolerance=1e-12)
 # warm starts (a converged state of a nearby problem): the initial residual is
 # small, so SER would keep the CFL near its start; start high instead

@@ -21,15 +21,7 @@ NEWTON = dict(cfl=5.0, ser=1, ser_exponent=1.5, cfl_max=1e10, gmres_rtol=1e-4, p
 
 
 def beta_field(case, seed, nblob=8, amp=0.2):
-    rng = np.random.default_rng(seed)
-    xc = 0.25 * (case.xv[:-1, :-1] + case.xv[1:, :-1] + case.xv[:-1, 1:] + case.xv[1:, 1:])
-    yc = 0.25 * (case.yv[:-1, :-1] + case.yv[1:, :-1] + case.yv[:-1, 1:] + case.yv[1:, 1:])
-    b = np.ones_like(xc)
-    for _ in range(nblob):
-        cx, cy = rng.uniform(-1.0, 1.0), rng.uniform(0.0, 0.1)
-        r = rng.uniform(0.05, 0.2)
-        b += rng.choice([-amp, amp]) * np.exp(-((xc - cx) ** 2 + (yc - cy) ** 2) / r ** 2)
-    return b
+    return cases.blob_beta(case, seed, nblob, amp)
 
 
 def test_adjoint_gradient_matches_central_differences():
