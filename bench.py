@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-closure", action="store_true", help="skip the ML-closure vs plain-SA leg")
     ap.add_argument("--adjoint-grid", default="705x321")
     ap.add_argument("--closure-grid", default="1409x641")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="MB: also time a plain device copy of this size (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -240,6 +242,21 @@ def main():
         "wall_s": wall,
         "l2": [float(x) for x in l2h],
     }
+    if a.copy_calibration > 0:
+        n = int(a.copy_calibration * 1e6 / 8)
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record()
+        for _ in range(10):
+            dst.copy_(src)
+        c1.record()
+        torch.cuda.synchronize()
+        ms = c0.elapsed_time(c1) / 10
+        out["copy_calibration"] = {"bytes_read": n * 8, "bytes_written": n * 8, "ms": ms,
+                                   "GBs": 2 * n * 8 / (ms * 1e-3) / 1e9}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, q0, a.cpu_seconds)
     eq.close()

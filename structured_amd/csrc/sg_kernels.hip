@@ -59,26 +59,34 @@ hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool 
   return hipGetLastError();
 }
 
-// Sum of the per-block partials (fixed assignment of partials to threads,
-// fixed butterfly and wave order, so the result is bitwise reproducible),
-// then sqrt.  One block, one pass over the partials.
-__global__ __launch_bounds__(256) void norm_finalize_kernel(const double* partial, int nblocks, int nv, double* l2) {
-  __shared__ double red[4][10];
+// Sum of the per-block partials, stored component-major (partial[k * nblocks
+// + b]): thread t adds blocks t, t + 1024, ... of every component with
+// coalesced loads, then a fixed butterfly per wavefront and a fixed order
+// over the 16 wavefronts, so the result is bitwise reproducible; then sqrt.
+__global__ __launch_bounds__(1024) void norm_finalize_kernel(const double* partial, int nblocks, int nv, double* l2) {
+  __shared__ double red[16][10];
   double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int b = threadIdx.x; b < nblocks; b += 256)
-    for (int k = 0; k < nv; ++k) acc[k] += partial[(size_t)b * nv + k];
+  for (int b = threadIdx.x; b < nblocks; b += 1024)
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
+      if (k < nv) acc[k] += partial[(size_t)k * nblocks + b];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int k = 0; k < nv; ++k) {
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
     double s = acc[k];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (lane == 0) red[wv][k] = s;
   }
   __syncthreads();
-  if ((int)threadIdx.x < nv) l2[threadIdx.x] = sqrt(red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
+  if ((int)threadIdx.x < nv) {
+    double s = 0.0;
+    for (int w = 0; w < 16; ++w) s += red[w][threadIdx.x];
+    l2[threadIdx.x] = sqrt(s);
+  }
 }
 
 hipError_t launch_norm_finalize(const double* partial, int nblocks, int nv, double* l2, hipStream_t s) {
-  hipLaunchKernelGGL(norm_finalize_kernel, dim3(1), dim3(256), 0, s, partial, nblocks, nv, l2);
+  hipLaunchKernelGGL(norm_finalize_kernel, dim3(1), dim3(1024), 0, s, partial, nblocks, nv, l2);
   return hipGetLastError();
 }
 

@@ -295,7 +295,7 @@ __device__ __forceinline__ SaTerms<S> sa_terms(S r, S n, S ml, const S* gu, cons
 // CL: closure kind of beta (ClosureKind); CLOSURE_MLP stages the hidden
 // layer of the network (and its tangent) in the same LDS area.
 template <class S, bool SA, int CL>
-__global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, DevBc bc, Field<S> q, OutField<S> out,
+__global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 4 : 1) void residual_kernel(DevGeom g, DevPhys ph, DevBc bc, Field<S> q, OutField<S> out,
                                                        double* partial, const double* beta_dot, DevClosure clo) {
   constexpr int NL = SA ? L_MAX : (L_K + 1);
   constexpr int NV = SA ? 5 : 4;
@@ -309,7 +309,16 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const int i0 = blockIdx.x * OX, j0 = blockIdx.y * OY;
+  // XCD-aware tile order: work-groups are handed to the 8 XCDs round-robin in
+  // linear order, so XCD x's k-th group takes the k-th tile of the x-th
+  // contiguous run of tiles; neighbouring tiles then share their halo rows in
+  // the same L2.  The map is a bijection on the tiles for any tile count.
+  int tile = blockIdx.y * gridDim.x + blockIdx.x;
+  {
+    const int N = gridDim.x * gridDim.y, x = tile & 7, k = tile >> 3, qn = N >> 3, rn = N & 7;
+    tile = (x < rn) ? x * (qn + 1) + k : rn * (qn + 1) + (x - rn) * qn + k;
+  }
+  const int i0 = (tile % gridDim.x) * OX, j0 = (tile / gridDim.x) * OY;
   const int nic = g.nic, njc = g.njc;
   const double nan = __builtin_nan("");
 
@@ -579,7 +588,7 @@ __global__ __launch_bounds__(NT) void residual_kernel(DevGeom g, DevPhys ph, Dev
     if (tid < 2 * NV) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
-      partial[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * 2 * NV + tid] = s;
+      partial[(size_t)tid * (gridDim.x * gridDim.y) + blockIdx.y * gridDim.x + blockIdx.x] = s;
     }
   }
 }
