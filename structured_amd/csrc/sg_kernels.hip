@@ -19,21 +19,22 @@ size_t residual_partial_count(const DevGeom& g) {
 
 template <class S, bool SA>
 static void launch_cl(dim3 grid, hipStream_t s, const DevGeom& g, const DevPhys& ph, const DevBc& bc, Field<S> fq,
-                      OutField<S> fo, double* partial, const double* beta_dot, const DevClosure& clo) {
+                      OutField<S> fo, double* partial, const double* beta_dot, const DevClosure& clo,
+                      NormOut no = NormOut{nullptr, nullptr}) {
   if constexpr (SA) {
     if (clo.kind == CLOSURE_ANALYTIC) {
       hipLaunchKernelGGL((residual_kernel<S, SA, CLOSURE_ANALYTIC>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial,
-                         beta_dot, clo);
+                         beta_dot, clo, no);
       return;
     }
     if (clo.kind == CLOSURE_MLP) {
       hipLaunchKernelGGL((residual_kernel<S, SA, CLOSURE_MLP>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial,
-                         beta_dot, clo);
+                         beta_dot, clo, no);
       return;
     }
   }
   hipLaunchKernelGGL((residual_kernel<S, SA, CLOSURE_FIELD>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial,
-                       beta_dot, clo);
+                       beta_dot, clo, no);
 }
 
 hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
@@ -41,11 +42,12 @@ hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc,
   dim3 grid = tile_grid(g);
   Field<double> fq{q};
   OutField<double> fo{r};
-  (void)counter;
-  if (sa) launch_cl<double, true>(grid, s, g, ph, bc, fq, fo, partial, nullptr, clo);
-  else launch_cl<double, false>(grid, s, g, ph, bc, fq, fo, partial, nullptr, DevClosure());
+  // with a counter the norms are finalised inside the residual kernel
+  const NormOut no{counter, (partial && counter) ? l2 : nullptr};
+  if (sa) launch_cl<double, true>(grid, s, g, ph, bc, fq, fo, partial, nullptr, clo, no);
+  else launch_cl<double, false>(grid, s, g, ph, bc, fq, fo, partial, nullptr, DevClosure(), no);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess || !partial || !l2) return e;
+  if (e != hipSuccess || !partial || !l2 || counter) return e;
   return launch_norm_finalize(partial, (int)residual_partial_count(g), 2 * (sa ? 5 : 4), l2, s);
 }
 

@@ -8,8 +8,9 @@ namespace sg {
 
 size_t residual_partial_count(const DevGeom& g);
 // partial: per-block norm partials (2 nv per block, or null); with l2
-// (device, 2 nv entries: the L2 norms of R, then of V R) a second kernel
-// finalises the norms.  counter is unused.
+// (device, 2 nv entries: the L2 norms of R, then of V R) the norms are
+// finalised by the residual kernel's last block when counter (a device
+// unsigned kept at 0 between launches) is given, else by a second kernel.
 hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
                            double* partial, unsigned* counter, double* l2, hipStream_t s,
                            const DevClosure& clo = DevClosure());

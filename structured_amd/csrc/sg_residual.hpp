@@ -292,11 +292,18 @@ __device__ __forceinline__ SaTerms<S> sa_terms(S r, S n, S ml, const S* gu, cons
   return t;
 }
 
+// Fused norm output: with l2 set, the last block finalises the norms.
+struct NormOut {
+  unsigned* counter;  // arrival count, 0 between launches
+  double* l2;         // 2 NV norms: ||R_k||_2, then ||V R_k||_2
+};
+
 // CL: closure kind of beta (ClosureKind); CLOSURE_MLP stages the hidden
 // layer of the network (and its tangent) in the same LDS area.
 template <class S, bool SA, int CL>
-__global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 4 : 1) void residual_kernel(DevGeom g, DevPhys ph, DevBc bc, Field<S> q, OutField<S> out,
-                                                       double* partial, const double* beta_dot, DevClosure clo) {
+__global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) void residual_kernel(DevGeom g, DevPhys ph, DevBc bc, Field<S> q, OutField<S> out,
+                                                       double* partial, const double* beta_dot, DevClosure clo,
+                                                       NormOut nout) {
   constexpr int NL = SA ? L_MAX : (L_K + 1);
   constexpr int NV = SA ? 5 : 4;
   constexpr int NLDS0 = NL * NPOS > 2 * NV * NT ? NL * NPOS : 2 * NV * NT;
@@ -578,6 +585,7 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 4 : 1) void resid
   // ---- block partial sums of R^2 (fixed order: wavefront butterfly, then waves)
   if (partial) {
     const int lane = tid & 63, wv = tid >> 6;
+    const int nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
     for (int k = 0; k < 2 * NV; ++k) {
       double s = sq[k];
@@ -588,7 +596,57 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 4 : 1) void resid
     if (tid < 2 * NV) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
-      partial[(size_t)tid * (gridDim.x * gridDim.y) + blockIdx.y * gridDim.x + blockIdx.x] = s;
+      // device-coherent store, acknowledged before the arrival count below
+      __hip_atomic_store(&partial[(size_t)tid * nb + bid], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (nout.l2) {
+      // The last block to arrive sums the partials of all blocks in a fixed
+      // order (so the norm does not depend on which block is last) and resets
+      // the arrival counter for the next launch.  Partials and counter are
+      // device-coherent atomics; each block's stores complete before its
+      // arrival, and the last block reads them after seeing every arrival.
+      __shared__ unsigned last;
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (tid == 0)
+        last = __hip_atomic_fetch_add(nout.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (unsigned)(nb - 1);
+      __syncthreads();
+      if (last) {
+        double acc[2 * NV];
+#pragma unroll
+        for (int k = 0; k < 2 * NV; ++k) acc[k] = 0.0;
+        // four rows of blocks per round, all loads of a round in flight together
+        for (int b0 = 0; b0 < nb; b0 += 4 * NT) {
+          double v[4][2 * NV];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int b = b0 + u * NT + tid;
+#pragma unroll
+            for (int k = 0; k < 2 * NV; ++k)
+              v[u][k] = b < nb ? __hip_atomic_load(&partial[(size_t)k * nb + b], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)
+                               : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < 2 * NV; ++k) acc[k] += v[u][k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2 * NV; ++k) {
+          double s = acc[k];
+          for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+          if (lane == 0) red[wv][k] = s;
+        }
+        __syncthreads();
+        if (tid < 2 * NV) {
+          double s = 0.0;
+          for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
+          nout.l2[tid] = sqrt(s);
+        }
+        if (tid == 0) __hip_atomic_store(nout.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
   }
 }

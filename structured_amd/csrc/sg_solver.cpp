@@ -430,8 +430,9 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     Timer tr;
     SG_TRY(launch_dt(ctx->g, ctx->ph.mu_inf, nv, q, ctx->dsmin.p, cfl, S->dtv.p, ctx->stream), "dt");
     SG_TRY(invert_dt(S->dtv.p, S->idt.p, ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->stream), "invert");
-    SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q, S->rhs.p, ctx->partial.p, nullptr, ctx->l2.p,
-                           ctx->stream, ctx->clo), "residual");
+    SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q, S->rhs.p, ctx->partial.p,
+                           reinterpret_cast<unsigned*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
+           "residual");
     SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
     SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
     t_res += tr.s();

@@ -36,6 +36,13 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("HIP library missing: build it with `python -m structured_amd.build`")
+        # torch wheels bundle their own HIP runtime under the same soname: load
+        # torch first so that the process keeps a single runtime whichever of
+        # the two touches the GPU first
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = C.CDLL(LIB_PATH)
         _lib.sg_last_error.restype = C.c_char_p
         _lib.sg_last_error.argtypes = [C.c_void_p]

@@ -100,3 +100,39 @@ def test_residual_repeatable_bitwise():
     r1, l1 = eq.calc_residual(q)
     r2, l2 = eq.calc_residual(q)
     assert np.array_equal(r1, r2) and np.array_equal(l1, l2)
+
+
+def test_fused_norm_over_many_launches():
+    """The residual kernel's last block finalises the L2 norms from every
+    block's partial sums: over 120 back-to-back launches on alternating states
+    (so a stale partial would show) each norm equals the norm of that
+    launch's R, computed on the host from the downloaded residual."""
+    import torch
+    case = cases.sa_plate(193, 129, dy1=1e-5)
+    eq = lib.Equation(case)
+    dev = torch.device("cuda:0")
+    nv, plane = eq.nvar, eq.plane
+    states = [case.perturbed_state(seed=s, amp=0.01 * (s + 1)) for s in range(3)]
+    qs = []
+    for q in states:
+        aos = torch.from_numpy(np.ascontiguousarray(q).reshape(-1)).to(dev)
+        qd = torch.zeros(nv * plane, dtype=torch.float64, device=dev)
+        eq.aos_to_soa_dev(aos.data_ptr(), qd.data_ptr())
+        qs.append(qd)
+    eq.synchronize()
+    expect = []
+    for q in states:
+        r, _ = eq.calc_residual(q)
+        expect.append(np.sqrt((r ** 2).sum(axis=(0, 1))))
+    rd = torch.zeros(nv * plane, dtype=torch.float64, device=dev)
+    l2s = torch.zeros((120, 16), dtype=torch.float64, device=dev)
+    for k in range(120):
+        eq.residual_dev_async(qs[k % 3].data_ptr(), rd.data_ptr(), l2s[k].data_ptr())
+    eq.synchronize()
+    got = l2s.cpu().numpy()
+    worst = 0.0
+    for k in range(120):
+        e = expect[k % 3]
+        worst = max(worst, float(np.abs(got[k, :nv] - e).max() / e.max()))
+    print("fused norm worst rel err over 120 launches: %.2e" % worst)
+    assert worst <= 1e-12
