@@ -1,20 +1,31 @@
-"""Benchmark of the fused FP64 residual on one MI355X.
+"""Benchmark of the fused FP64 residual on one MI355X (plus the adjoint
+gradient and the learned-closure solve).
 
-Workload: turbulent flat plate with Spalart-Allmaras, 545 x 385 vertices
-(544 x 384 cells, 5 unknowns per cell), the residual configuration named by
-the project's headline metric.  One step = one residual evaluation of the
-whole grid (fused residual kernel with the per-block norm partial sums, then
-the norm finalisation) on a seeded perturbed freestream state already
-resident in HBM.  Steps rotate over independent state/residual buffer sets
-totalling > 512 MB so that the 256 MB Infinity Cache cannot hold the working
-set.
+Headline (value): turbulent flat plate with Spalart-Allmaras, 545 x 385
+vertices (544 x 384 cells, 5 unknowns per cell).  One step = one residual
+evaluation of the whole grid including its L2 norms (the fused residual kernel
+whose last block finalises the norms) on a seeded perturbed freestream state
+already resident in HBM.  Steps rotate over independent state/residual buffer
+sets totalling > 512 MB so that the 256 MB Infinity Cache cannot hold the
+working set.  Timed with HIP events on the library's own stream.
 
-Prints one JSON line.  The roofline entry uses the algorithmic bytes of the
-SA residual (q 5 + metrics 5 + wall distance 1 + beta 1 + R 5 = 17 doubles =
-136 B per cell) over the residual kernel's average duration, timed with HIP
-events on the library's own stream.  cpu_baseline times the CPU oracle (a
-C++ restatement of the reference's residual with the reference's OpenMP
-loops) on the box's host cores on the same grid and state.
+roofline: the residual kernel alone (launches without the norm), algorithmic
+bytes of the SA residual (q 5 + normals 4 + volume 1 + wall distance 1 +
+beta 1 + R 5 = 17 doubles = 136 B per cell) over its average duration;
+traffic is the calibrated HBM bytes per launch from the committed counter
+profile of the same grid.  cpu_baseline times the CPU oracle (a C++
+restatement of the reference's residual with the reference's OpenMP loops) on
+the box's host cores on the same grid and state.
+
+adjoint_gradient: wall time of one adjoint beta-gradient of the wall-Cf
+objective on the 705 x 321 channel bump (beta = 1 + 8 seeded blobs, target Cf
+from a second seeded beta*), after converging the primal states.
+
+ml_closure_vs_sa: plain-SA solve against the solve with the closure network
+fused into the SA source, both from freestream on the 1409 x 641 bump
+(h 0.065, w 0.25, Re 5e6).
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -126,6 +137,19 @@ def closure_leg(grid):
     return out
 
 
+def measured_traffic(ni, nj):
+    """HBM bytes per launch of the residual kernel from the committed
+    counter profile (FETCH_SIZE / WRITE_SIZE passes scaled by a copy of known
+    size), when it was taken on this grid; else None."""
+    path = os.path.join(ROOT, "profiles", "residual_sa_%dx%d_traffic.md" % (ni, nj))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(case, q, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as of
@@ -235,7 +259,8 @@ def main():
                    "cells": cells, "unknowns_per_cell": nv, "rotating_sets": nsets,
                    "rotating_bytes": nsets * set_bytes},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "residual_kernel<double,SA>",
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(a.ni, a.nj),
+                     "kernel": "residual_kernel<double,SA>",
                      "kernel_ms": ms_kernel, "bytes_per_cell": bytes_cell},
         "jv": {"ms": ms_jv, "mcells_s": cells / (ms_jv * 1e-3) / 1e6,
                "achieved_GBs_at_176B_cell": 176 * cells / (ms_jv * 1e-3) / 1e9},

@@ -136,3 +136,100 @@ def predict(case, net, q0=None, newton=None):
     _, _, cf, _ = eq.wall(q)
     eq.close()
     return q, rep, t, cf
+
+
+# ---------------------------------------------------------------- experiment
+def _bumps(n, seed):
+    rng = np.random.default_rng(seed)
+    return [(float(h), float(w)) for h, w in zip(rng.uniform(0.02, 0.08, n), rng.uniform(0.1, 0.3, n))]
+
+
+def stage_invert(args):
+    """Invert a slice of the seeded training bumps; saves features and
+    inverted beta (lower half of the grid, float32) per case."""
+    import json
+    import os
+    os.makedirs(args.out, exist_ok=True)
+    coef = truth_coef(seed=args.truth_seed)
+    ni, nj = (int(v) for v in args.train_grid.split("x"))
+    bumps = _bumps(args.ncases, args.case_seed)
+    lo, hi = (int(v) for v in args.cases.split(":"))
+    for k in range(lo, min(hi, len(bumps))):
+        h, w = bumps[k]
+        case = bump_case(ni, nj, h, w, re=3e6)
+        r = run_case(case, coef, invert_iterations=args.iterations, verbose=True)
+        half = case.njc // 2
+        np.savez_compressed(os.path.join(args.out, "case_%02d.npz" % k),
+                            features=r["features"][:, :half].astype(np.float32),
+                            beta_inv=r["beta_inv"][:, :half].astype(np.float32),
+                            beta_true=r["beta_true"][:, :half].astype(np.float32),
+                            features_true=r["features_true"][:, :half].astype(np.float32))
+        rep = r["inv_report"]
+        summary = dict(case=k, height=h, width=w, sa_s=r["sa_s"], true_s=r["true_s"], inv_s=r["inv_s"],
+                       inv_iterations=rep["iterations"], J0=rep["J0"], J=rep["J"],
+                       cf_err_sa=cf_error(r["cf_sa"], r["cf_true"]), cf_err_inv=cf_error(r["cf_inv"], r["cf_true"]),
+                       sa_converged=bool(r["sa_report"]["converged"]),
+                       true_converged=bool(r["true_report"]["converged"]))
+        with open(os.path.join(args.out, "case_%02d.json" % k), "w") as f:
+            json.dump(summary, f)
+        print("case", json.dumps(summary), flush=True)
+
+
+def stage_test(args):
+    """Train on every saved case, then on the unseen bump compare plain SA,
+    the learned closure and the full inversion against the truth."""
+    import glob
+    import json
+    import os
+    coef = truth_coef(seed=args.truth_seed)
+    files = sorted(glob.glob(os.path.join(args.data, "case_*.npz")))
+    X = np.concatenate([np.load(f)["features"].reshape(-1, 5) for f in files]).astype(np.float32)
+    T = np.concatenate([np.load(f)["beta_inv"].reshape(-1) - 1.0 for f in files]).astype(np.float32)
+    print("training set: %d cases, %d samples" % (len(files), len(X)), flush=True)
+    net, hist, t_train = train(X, T, epochs=args.epochs, batch=65536, lr=1e-3, seed=args.net_seed)
+    print("trained %d epochs in %.2fs: loss %.3e -> %.3e" % (args.epochs, t_train, hist[0], hist[-1]), flush=True)
+    ni, nj = (int(v) for v in args.test_grid.split("x"))
+    case = bump_case(ni, nj, 0.065, 0.25, re=5e6)
+    res = run_case(case, coef, invert_iterations=args.iterations, verbose=True)
+    q_ml, rep_ml, t_ml, cf_ml = predict(case, net)
+    out = dict(train_cases=len(files), train_samples=int(len(X)), train_s=t_train, train_loss=[float(hist[0]),
+                                                                                              float(hist[-1])],
+               test_grid=args.test_grid, sa_s=res["sa_s"], truth_s=res["true_s"], inversion_s=res["inv_s"],
+               inversion_iterations=res["inv_report"]["iterations"], ml_solve_s=t_ml,
+               ml_solve_ok=closure_solve_ok(rep_ml), ml_newton=rep_ml["iterations"],
+               cf_err_sa_vs_truth=cf_error(res["cf_sa"], res["cf_true"]),
+               cf_err_inv_vs_truth=cf_error(res["cf_inv"], res["cf_true"]),
+               cf_err_ml_vs_truth=cf_error(cf_ml, res["cf_true"]),
+               cf_err_ml_vs_inversion=cf_error(cf_ml, res["cf_inv"]),
+               inversion_over_ml_time=res["inv_s"] / t_ml)
+    print("result", json.dumps(out), flush=True)
+    if args.out:
+        os.makedirs(args.out, exist_ok=True)
+        with open(os.path.join(args.out, "test.json"), "w") as f:
+            json.dump(out, f)
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="field inversion and machine learning of the SA beta closure")
+    ap.add_argument("stage", choices=["invert", "test"])
+    ap.add_argument("--ncases", type=int, default=24)
+    ap.add_argument("--cases", default="0:24", help="slice of the training bumps to invert")
+    ap.add_argument("--train-grid", default="385x193")
+    ap.add_argument("--test-grid", default="1409x641")
+    ap.add_argument("--iterations", type=int, default=50)
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--truth-seed", type=int, default=1)
+    ap.add_argument("--case-seed", type=int, default=7)
+    ap.add_argument("--net-seed", type=int, default=0)
+    ap.add_argument("--data", default="fiml_data")
+    ap.add_argument("--out", default="fiml_data")
+    args = ap.parse_args(argv)
+    if args.stage == "invert":
+        stage_invert(args)
+    else:
+        stage_test(args)
+
+
+if __name__ == "__main__":
+    main()
