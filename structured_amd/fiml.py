@@ -65,7 +65,7 @@ def cf_error(cf, ref):
     return float(np.linalg.norm(cf - ref) / max(np.linalg.norm(ref), 1e-300))
 
 
-def run_case(case, coef, invert_iterations=50, newton=None, verbose=False):
+def run_case(case, coef, invert_iterations=50, newton=None, verbose=False, resolve=None):
     """Truth, baseline and inversion on one case; returns a dict with the
     states, beta fields, wall Cf and timings."""
     newton = dict(NEWTON, **(newton or {}))
@@ -73,6 +73,9 @@ def run_case(case, coef, invert_iterations=50, newton=None, verbose=False):
     out = {"case": case.name}
     q_sa, rep, t = timed_solve(eq, case.freestream_state(), **newton)
     out.update(q_sa=q_sa, sa_report=rep, sa_s=t)
+    if verbose:
+        print("%s: plain SA %.2fs (%d Newton steps, converged %d)" % (case.name, t, rep["iterations"],
+                                                                     rep["converged"]), flush=True)
     eq.set_closure(1, coef=coef)
     q_true, rep, t = timed_solve(eq, q_sa, **dict(newton, cfl=WARM["cfl"]))
     out.update(q_true=q_true, true_report=rep, true_s=t)
@@ -85,8 +88,8 @@ def run_case(case, coef, invert_iterations=50, newton=None, verbose=False):
         eq.close()
         return out
     t0 = time.perf_counter()
-    q_inv, beta_inv, irep, jh = eq.invert(q_sa, cf_true, iterations=invert_iterations, newton=dict(RESOLVE),
-                                          adjoint=ADJOINT)
+    q_inv, beta_inv, irep, jh = eq.invert(q_sa, cf_true, iterations=invert_iterations,
+                                          newton=dict(resolve or RESOLVE), adjoint=ADJOINT)
     out.update(q_inv=q_inv, beta_inv=beta_inv, inv_report=irep, inv_hist=jh, inv_s=time.perf_counter() - t0)
     _, _, cf_inv, _ = eq.wall(q_inv)
     feats, _ = eq.closure_eval(q_inv)
@@ -190,7 +193,8 @@ def stage_test(args):
     print("trained %d epochs in %.2fs: loss %.3e -> %.3e" % (args.epochs, t_train, hist[0], hist[-1]), flush=True)
     ni, nj = (int(v) for v in args.test_grid.split("x"))
     case = bump_case(ni, nj, 0.065, 0.25, re=5e6)
-    res = run_case(case, coef, invert_iterations=args.iterations, verbose=True)
+    # the inversion's primal re-solves report every Newton step (progress)
+    res = run_case(case, coef, invert_iterations=args.iterations, verbose=True, resolve=dict(RESOLVE, verbose=1))
     q_ml, rep_ml, t_ml, cf_ml = predict(case, net)
     out = dict(train_cases=len(files), train_samples=int(len(X)), train_s=t_train, train_loss=[float(hist[0]),
                                                                                               float(hist[-1])],
