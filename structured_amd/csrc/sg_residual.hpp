@@ -183,7 +183,7 @@ __device__ __forceinline__ void roe_flux(double nx, double ny, S rl, S ul, S vl,
   S uv = 0.5 * (uav * uav + vav * vav);
   S cav = ssqrt(gm1 * (hav - uv));
   S d1 = rr - rl, d2 = ur - ul, d3 = vr - vl, d4 = pr - pl;
-  const double dr = sqrt(nx * nx + ny * ny), idr = frcp(dr), r1 = nx * idr, r2 = ny * idr;
+  const double dr = fsqrt(nx * nx + ny * ny), idr = frcp(dr), r1 = nx * idr, r2 = ny * idr;
   S uu = r1 * uav + r2 * vav, c2 = cav * cav, c2i = srcp(c2);
   S auu = sabs(uu), aupc = sabs(uu + cav), aumc = sabs(uu - cav);
   S uul = r1 * ul + r2 * vl, uur = r1 * ur + r2 * vr;
@@ -279,7 +279,7 @@ __device__ __forceinline__ SaTerms<S> sa_terms(S r, S n, S ml, const S* gu, cons
     S gg = rr + kCw2 * (r6 - rr);
     S g2 = gg * gg, g6 = g2 * g2 * g2;
     const double cw3_6 = kCw3 * kCw3 * kCw3 * kCw3 * kCw3 * kCw3;
-    S fw = gg * spow(sdiv(S(1.0 + cw3_6), g6 + cw3_6), 1.0 / 6.0);
+    S fw = gg * spow16(sdiv(S(1.0 + cw3_6), g6 + cw3_6));
     t.P1 = kCb1 * stil * r * n;
     t.D = cw1 * fw * r * (n * id) * (n * id);
   } else {

@@ -83,10 +83,39 @@ __device__ __forceinline__ double val(Dual x) { return x.v; }
 __device__ __forceinline__ double tan_(double) { return 0.0; }
 __device__ __forceinline__ double tan_(Dual x) { return x.d; }
 
-__device__ __forceinline__ double ssqrt(double x) { return sqrt(x); }
+// Square root of a positive normal x from the hardware reciprocal square
+// root estimate: y = x r, h = r / 2, one coupled Newton (Goldschmidt) step,
+// then two residual corrections y + (x - y^2) h (Markstein).  No scaling of
+// tiny or huge inputs and no special cases: 0, infinities and negatives are
+// not handled (the callers pass squared magnitudes bounded away from zero).
+__device__ __forceinline__ double fsqrt(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  double y = x * r, h = 0.5 * r;
+  const double e = __builtin_fma(-y, h, 0.5);
+  y = __builtin_fma(y, e, y);
+  h = __builtin_fma(h, e, h);
+  double d = __builtin_fma(-y, y, x);
+  y = __builtin_fma(d, h, y);
+  d = __builtin_fma(-y, y, x);
+  return __builtin_fma(d, h, y);
+}
+
+// z^(1/6) for z > 0 without pow: seed from the single-precision log2 / exp2
+// units, then two Newton steps on y^6 = z: y <- y (5 + z / y^6) / 6.
+__device__ __forceinline__ double pow16(double z) {
+  double y = (double)__builtin_amdgcn_exp2f((1.0f / 6.0f) * __builtin_amdgcn_logf((float)z));
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double y2 = y * y;
+    y = y * (5.0 + z * frcp(y2 * y2 * y2)) * (1.0 / 6.0);
+  }
+  return y;
+}
+
+__device__ __forceinline__ double ssqrt(double x) { return fsqrt(x); }
 __device__ __forceinline__ Dual ssqrt(Dual x) {
-  double r = sqrt(x.v);
-  return Dual(r, x.d / (2.0 * r));
+  double r = fsqrt(x.v);
+  return Dual(r, x.d * frcp(2.0 * r));
 }
 __device__ __forceinline__ double sabs(double x) { return fabs(x); }
 __device__ __forceinline__ Dual sabs(Dual x) { return x.v < 0.0 ? -x : x; }
@@ -100,6 +129,12 @@ __device__ __forceinline__ double spow(double x, double e) { return pow(x, e); }
 __device__ __forceinline__ Dual spow(Dual x, double e) {
   double r = pow(x.v, e);
   return Dual(r, e * pow(x.v, e - 1.0) * x.d);
+}
+// z^(1/6) (SA destruction f_w)
+__device__ __forceinline__ double spow16(double z) { return pow16(z); }
+__device__ __forceinline__ Dual spow16(Dual z) {
+  const double y = pow16(z.v);
+  return Dual(y, (1.0 / 6.0) * y * frcp(z.v) * z.d);
 }
 __device__ __forceinline__ double smin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ Dual smin(Dual a, double b) { return a.v < b ? a : Dual(b); }
