@@ -64,11 +64,19 @@ typedef struct sg_bc {
   double u, v, T;
 } sg_bc;
 
+/* Convective flux (src/model/flux.cpp): ConvectiveFluxRoe (:51-146),
+ * ConvectiveFluxAUSM (:149-224). */
+#define SG_FLUX_ROE 0
+#define SG_FLUX_AUSM 1
+
 typedef struct sg_opts {
   int order;       /* MUSCL order, 1 or 2 (solver.order)                    */
   int turbulence;  /* 0: laminar (4 unknowns), 1: Spalart-Allmaras (5)      */
   int tail;        /* geometry.tail: surface cells are [tail-1, ni-tail)    */
-  int reserved;
+  int flux;        /* solver.flux: SG_FLUX_ROE (0) or SG_FLUX_AUSM (1); the
+                      reference's config default is "ausm"
+                      (src/utils/config.cpp:54, selected at
+                      src/model/eulerequation.cpp:122-127)                   */
   double dpdx, dpdy;  /* [source] body force                                */
 } sg_opts;
 
@@ -243,6 +251,45 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* opts, sg_solve_re
 /* Per-iteration L2 norms (nvar per entry) and CFL of the last sg_solve;
  * returns the number of entries recorded. */
 int sg_solve_history(sg_ctx* ctx, double* l2_hist, double* cfl_hist, int max_entries);
+
+/* ---- explicit pseudo-time stepping: the build of Solver::step without
+ *      ADOL-C, src/solver/solver.cpp:103-121, with the update kernels
+ *      update_forward_euler / update_rk4 (:12-26) ---- */
+#define SG_SCHEME_FORWARD_EULER 0   /* solver.scheme = "forward_euler" */
+#define SG_SCHEME_RK4_JAMESON 1     /* solver.scheme = "rk4_jameson"   */
+
+/* One step on a device state (SoA, as sg_residual_dev): dt = calc_dt(q, cfl)
+ * per cell (eulerequation.cpp:237-259), then
+ *   forward Euler:  q <- q + R(q) dt;
+ *   RK4-Jameson:    q_s <- q + R(q_{s-1}) dt / (4 - s), s = 0..3, q_{-1} = q,
+ *                   and q <- q_3.
+ * l2 (host, nvar entries, optional): ||R_k||_2 of the last stage's residual,
+ * the norm Solver::step reports (solver.cpp:125-134).  SG_ERR_NAN when that
+ * norm is not finite (q is updated all the same, as in the reference). */
+int sg_step_explicit_dev(sg_ctx* ctx, double* q_dev, int scheme, double cfl, double* l2);
+
+typedef struct sg_explicit_opts {
+  int scheme;               /* SG_SCHEME_*                                      */
+  int iteration_max;        /* solver.iteration_max                             */
+  double cfl;               /* solver.cfl                                       */
+  int cfl_ramp;             /* solver.cfl_ramp                                  */
+  int cfl_ramp_iteration;   /* solver.cfl_ramp_iteration                        */
+  double cfl_ramp_exponent; /* solver.cfl_ramp_exponent                         */
+  double tolerance;         /* 0 (the reference's test is disabled,
+                               solver.cpp:145); > 0 stops once every ||R_k||_2
+                               is below it                                    */
+  int verbose;
+  int reserved;
+} sg_explicit_opts;
+
+/* Solver::solve with an explicit scheme (solver.cpp:197-236): steps
+ * counter = 0, 1, ... each update q, and the loop ends after the step whose
+ * counter exceeds iteration_max (iteration_max + 2 updates), the CFL ramping
+ * as CFL <- min(CFL^e, 1e12) once counter > cfl_ramp_iteration.  q_aos: host
+ * state, updated in place.  l2_hist (optional): nvar norms per step, room for
+ * iteration_max + 2 steps.  report->iterations is the number of steps. */
+int sg_solve_explicit(sg_ctx* ctx, double* q_aos, const sg_explicit_opts* opts, sg_solve_report* report,
+                      double* l2_hist);
 
 /* ---- field inversion (no counterpart in the reference): L-BFGS on
  *      J(beta) = F(q(beta)) + reg * sum (beta - 1)^2 with projected Armijo

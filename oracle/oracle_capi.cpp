@@ -31,6 +31,7 @@ void make_setup(Setup& s, int ni, int nj, const double* xv, const double* yv, co
   s.prm.p_inf = fs->p_inf; s.prm.T_inf = fs->T_inf; s.prm.mu_inf = fs->mu_inf;
   s.prm.pr_inf = fs->pr_inf; s.prm.aoa = fs->aoa_deg * M_PI / 180.0; s.prm.nut_inf = fs->nut_inf;
   s.prm.dpdx = o->dpdx; s.prm.dpdy = o->dpdy; s.prm.order = o->order; s.prm.sa = o->turbulence ? 1 : 0;
+  s.prm.flux = o->flux;
   s.bcs.clear();
   for (int b = 0; b < nbc; ++b) {
     Bc c;

@@ -7,6 +7,8 @@
 //   ref_harness <config.inp> dt   <state.bin> <cfl> <dt.bin>
 //   ref_harness <config.inp> surf <state.bin>          (writes <label>.surface)
 //   ref_harness <config.inp> grad <state.bin> <out.bin> (grad_{u,v,T}_{chi,eta})
+//   ref_harness <config.inp> solve <state.bin> <out.bin> (Solver::solve from the
+//              state with the config's explicit scheme; writes the final q)
 //
 // States and outputs are raw native-endian float64 in the reference layout
 // q[i][j][k] (the restart format of src/utils/io.cpp:134-152).  Run it from
@@ -20,6 +22,7 @@
 #include "eulerequation.h"
 #include "mesh.h"
 #include "solution.h"
+#include "solver.h"
 
 static std::vector<double> read_bin(const char* path, size_t n) {
   std::vector<double> v(n);
@@ -64,6 +67,13 @@ int main(int argc, char** argv) {
     std::ofstream f(argv[4], std::ios::binary);
     for (auto* a : {&e->grad_u_chi, &e->grad_v_chi, &e->grad_T_chi, &e->grad_u_eta, &e->grad_v_eta, &e->grad_T_eta})
       f.write(reinterpret_cast<const char*>(a->data()), (std::streamsize)(a->size() * sizeof(double)));
+  } else if (mode == "solve") {
+    // EulerEquation::initialize leaves q_tmp = q (src/model/eulerequation.cpp:282-288)
+    for (size_t m = 0; m < n; ++m) sol->q_tmp.data()[m] = q[m];
+    auto solver = std::make_shared<Solver<double, double>>(config);
+    solver->add_mesh(mesh);
+    solver->solve();
+    write_bin(argv[4], sol->q.data(), n);
   } else {
     std::fprintf(stderr, "unknown mode\n");
     return 2;

@@ -203,6 +203,7 @@ struct Params {
   double dpdx = 0.0, dpdy = 0.0;
   int order = 2;      // MUSCL order: 1 or 2
   int sa = 0;         // 0: laminar, nq=4; 1: Spalart-Allmaras, 5 unknowns
+  int flux = 0;       // convective flux: 0 Roe, 1 AUSM (solver.flux)
   bool viscous() const { return mu_inf > 1e-15; }   // src/utils/config.cpp:41
   int nvar() const { return sa ? 5 : 4; }
 };
@@ -518,6 +519,41 @@ struct Residual {
     if (with_sa) f[4] = 0.5 * (f[0] * (ntl + ntr) - fabs(f[0]) * (ntr - ntl));
   }
 
+  // ConvectiveFluxAUSM::evaluate, src/model/flux.cpp:149-224 (Liou-Steffen
+  // AUSM: split Mach numbers and pressures, upwinded on the interface Mach
+  // number).  With SA, rho*nu~ is carried first-order upwind by the AUSM mass
+  // flux, as for Roe.
+  static T split_mach(T M, double sgn) {  // M+ (sgn=+1) / M- (sgn=-1)
+    if (fabs(M) <= 1.0) return sgn * 0.25 * (M + sgn) * (M + sgn);
+    return 0.5 * (M + sgn * fabs(M));
+  }
+  static T split_pres(T M, T p, double sgn) {  // P+ / P-
+    if (fabs(M) <= 1.0) return 0.25 * p * (M + sgn) * (M + sgn) * (2.0 - sgn * M);
+    return 0.5 * p * (M + sgn * fabs(M)) / M;
+  }
+  static void ausm(double nx, double ny, T rl, T ul, T vl, T pl, T rr, T ur, T vr, T pr, T ntl, T ntr, bool with_sa, T* f) {
+    const double ogm1 = 1.0 / (kGamma - 1.0);
+    const double ds = std::sqrt(nx * nx + ny * ny);
+    T al = sqrt(kGamma * pl / rl), ar = sqrt(kGamma * pr / rr);
+    T Ml = ((ul * nx + vl * ny) / ds) / al, Mr = ((ur * nx + vr * ny) / ds) / ar;
+    T hl = (pl * ogm1 + rl * 0.5 * (ul * ul + vl * vl) + pl) / rl;
+    T hr = (pr * ogm1 + rr * 0.5 * (ur * ur + vr * vr) + pr) / rr;
+    T mh = split_mach(Ml, 1.0) + split_mach(Mr, -1.0);
+    T ph = split_pres(Ml, pl, 1.0) + split_pres(Mr, pr, -1.0);
+    const bool up = mh >= 0.0;
+    T m = (up ? rl * al : rr * ar) * mh * ds;
+    f[0] = m;
+    f[1] = m * (up ? ul : ur) + ph * nx;
+    f[2] = m * (up ? vl : vr) + ph * ny;
+    f[3] = m * (up ? hl : hr);
+    if (with_sa) f[4] = 0.5 * (f[0] * (ntl + ntr) - fabs(f[0]) * (ntr - ntl));
+  }
+
+  void conv_flux(double nx, double ny, T rl, T ul, T vl, T pl, T rr, T ur, T vr, T pr, T ntl, T ntr, T* f) const {
+    if (prm.flux == 1) ausm(nx, ny, rl, ul, vl, pl, rr, ur, vr, pr, ntl, ntr, prm.sa, f);
+    else roe(nx, ny, rl, ul, vl, pl, rr, ur, vr, pr, ntl, ntr, prm.sa, f);
+  }
+
   // DiffusiveFluxGreenGauss::evaluate, src/model/flux.cpp:12-48; with SA the
   // nu~ diffusion (1/sigma) * eta_bar * grad(nu~).n is added as component 4.
   void viscous_flux(double nx, double ny, const T* gu, const T* gv, const T* gT, T ub, T vb, T m, T k, const T* gn, T et, T* f) const {
@@ -685,16 +721,16 @@ struct Residual {
     for (size_t i = 0; i < ni; ++i)
       for (size_t j = 0; j < njc; ++j) {
         T f[5];
-        roe(nc(i, j, 0), nc(i, j, 1), ql[0](i, j), ql[1](i, j), ql[2](i, j), ql[3](i, j),
-            qr[0](i, j), qr[1](i, j), qr[2](i, j), qr[3](i, j), ql[4](i, j), qr[4](i, j), prm.sa, f);
+        conv_flux(nc(i, j, 0), nc(i, j, 1), ql[0](i, j), ql[1](i, j), ql[2](i, j), ql[3](i, j),
+                  qr[0](i, j), qr[1](i, j), qr[2](i, j), qr[3](i, j), ql[4](i, j), qr[4](i, j), f);
         for (size_t k = 0; k < nv; ++k) fc(i, j, k) = f[k];
       }
 #pragma omp parallel for
     for (size_t i = 0; i < nic; ++i)
       for (size_t j = 0; j < nj; ++j) {
         T f[5];
-        roe(ne(i, j, 0), ne(i, j, 1), el[0](i, j), el[1](i, j), el[2](i, j), el[3](i, j),
-            er[0](i, j), er[1](i, j), er[2](i, j), er[3](i, j), el[4](i, j), er[4](i, j), prm.sa, f);
+        conv_flux(ne(i, j, 0), ne(i, j, 1), el[0](i, j), el[1](i, j), el[2](i, j), el[3](i, j),
+                  er[0](i, j), er[1](i, j), er[2](i, j), er[3](i, j), el[4](i, j), er[4](i, j), f);
         for (size_t k = 0; k < nv; ++k) fe(i, j, k) = f[k];
       }
     for (size_t m = 0; m < nic * njc * nv; ++m) rhs[m] = T(0.0);

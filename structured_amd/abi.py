@@ -17,8 +17,11 @@ class Bc(C.Structure):
                 ("u", C.c_double), ("v", C.c_double), ("T", C.c_double)]
 
 
+FLUXES = {"roe": 0, "ausm": 1}
+
+
 class Opts(C.Structure):
-    _fields_ = [("order", C.c_int), ("turbulence", C.c_int), ("tail", C.c_int), ("reserved", C.c_int),
+    _fields_ = [("order", C.c_int), ("turbulence", C.c_int), ("tail", C.c_int), ("flux", C.c_int),
                 ("dpdx", C.c_double), ("dpdy", C.c_double)]
 
 
@@ -77,6 +80,24 @@ def newton_opts(**kw):
         raise KeyError("unknown Newton options: %s" % sorted(unknown))
     d.update(kw)
     return NewtonOpts(**d)
+
+
+SCHEMES = {"forward_euler": 0, "rk4_jameson": 1}
+
+
+class ExplicitOpts(C.Structure):
+    _fields_ = [("scheme", C.c_int), ("iteration_max", C.c_int), ("cfl", C.c_double), ("cfl_ramp", C.c_int),
+                ("cfl_ramp_iteration", C.c_int), ("cfl_ramp_exponent", C.c_double), ("tolerance", C.c_double),
+                ("verbose", C.c_int), ("reserved", C.c_int)]
+
+
+def explicit_opts(scheme="forward_euler", iteration_max=1, cfl=1.0, cfl_ramp=False, cfl_ramp_iteration=20,
+                  cfl_ramp_exponent=1.1, tolerance=0.0, verbose=0):
+    """The reference's [solver] defaults (src/utils/config.cpp:46-61)."""
+    return ExplicitOpts(scheme=SCHEMES[scheme] if isinstance(scheme, str) else int(scheme),
+                        iteration_max=iteration_max, cfl=cfl, cfl_ramp=int(bool(cfl_ramp)),
+                        cfl_ramp_iteration=cfl_ramp_iteration, cfl_ramp_exponent=cfl_ramp_exponent,
+                        tolerance=tolerance, verbose=verbose, reserved=0)
 
 
 class AdjointOpts(C.Structure):

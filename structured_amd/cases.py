@@ -30,6 +30,7 @@ class Case:
     tail: int = 1
     dpdx: float = 0.0
     dpdy: float = 0.0
+    flux: str = "roe"  # solver.flux: "roe" or "ausm"
     meta: dict = field(default_factory=dict)
 
     @property
@@ -171,8 +172,9 @@ def channel_example(path_grid: str | None = None) -> Case:
     return Case("channel_3x101", xv, yv, fs, bcs, order=2, turbulence=0, tail=1, dpdx=-0.02592)
 
 
-def to_toml(case: Case, grid_file: str, label: str = "case") -> str:
-    """Reference-format configuration (src/utils/config.cpp:31-87 keys)."""
+def to_toml(case: Case, grid_file: str, label: str = "case", solver: dict | None = None) -> str:
+    """Reference-format configuration (src/utils/config.cpp:31-87 keys).
+    `solver`: extra [solver] keys (scheme, cfl, iteration_max, cfl_ramp, ...)."""
     f = case.fs
     out = ["[geometry]", 'filename = "%s"' % grid_file, "ni = %d" % case.ni, "nj = %d" % case.nj,
            "tail = %d" % case.tail, 'format = "p3d"', "",
@@ -181,8 +183,19 @@ def to_toml(case: Case, grid_file: str, label: str = "case") -> str:
         out.append("%s = %.17e" % (k, f[k]))
     out.append("aoa = %.17e" % f.get("aoa_deg", 0.0))
     out += ["", "[source]", "dpdx = %.17e" % case.dpdx, "dpdy = %.17e" % case.dpdy, "",
-            "[solver]", "order = %d" % case.order, "lhs_order = %d" % case.order, 'flux = "roe"',
-            "iteration_max = 1", "", "[io]", 'label = "%s"' % label, "restart = false", ""]
+            "[solver]", "order = %d" % case.order, "lhs_order = %d" % case.order, 'flux = "%s"' % case.flux]
+    sv = {"iteration_max": 1, **(solver or {})}
+    for k, v in sv.items():
+        if isinstance(v, bool):
+            out.append("%s = %s" % (k, "true" if v else "false"))
+        elif isinstance(v, str):
+            out.append('%s = "%s"' % (k, v))
+        elif isinstance(v, int):
+            out.append("%s = %d" % (k, v))
+        else:
+            out.append("%s = %.17e" % (k, v))
+    out += ["", "[io]", 'label = "%s"' % label, "restart = false", "stdout_frequency = 1000000",
+            "fileout_frequency = 1000000", ""]
     for b in case.bcs:
         out += ["[[boundary]]", 'name = ""', 'type = "%s"' % b["type"], 'face = "%s"' % b["face"],
                 "start = %d" % b.get("start", 0), "end = %d" % b.get("end", 0)]

@@ -347,6 +347,8 @@ int sg_create(int ni, int nj, const double* xv, const double* yv, const sg_frees
   if (ni < 3 || nj < 3 || !xv || !yv || !fs || !opts || (nbc > 0 && !bcs) || nbc < 0)
     return sg_fail(nullptr, SG_ERR_ARG, "bad shape or null argument (ni, nj >= 3 required)");
   if (opts->order != 1 && opts->order != 2) return sg_fail(nullptr, SG_ERR_ARG, "order must be 1 or 2");
+  if (opts->flux != SG_FLUX_ROE && opts->flux != SG_FLUX_AUSM)
+    return sg_fail(nullptr, SG_ERR_ARG, "flux must be SG_FLUX_ROE or SG_FLUX_AUSM");
   sg_ctx* ctx = new sg_ctx();
   ctx->fs = *fs;
   ctx->opts = *opts;
@@ -412,6 +414,7 @@ int sg_create(int ni, int nj, const double* xv, const double* yv, const sg_frees
   ph.eps_chi = std::pow(10.0 / G.nic, 3);               // src/model/reconstruction.cpp:62-63
   ph.eps_eta = std::pow(10.0 / G.njc, 3);
   ph.dpdx = opts->dpdx; ph.dpdy = opts->dpdy; ph.order = opts->order; ph.viscous = viscous ? 1 : 0;
+  ph.flux = opts->flux == SG_FLUX_AUSM ? FLUX_AUSM : FLUX_ROE; ph.pad_ = 0;
   // boundary nodes
   {
     const size_t nb = ctx->bcc.nodes.size() * sizeof(BcNode), nm = ctx->bcc.ghost_map.size() * sizeof(int);

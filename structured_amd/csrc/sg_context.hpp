@@ -83,6 +83,7 @@ struct sg_ctx {
   sg::DevClosure clo;              // beta closure applied by every residual / J.v launch
   sg::DevBuf clo_params, clo_w2;   // MLP parameters (FP32) and W2 (BF16), raw storage
   sg::DevBuf aux;                  // features (5 planes) + beta, for sg_closure_eval
+  sg::DevBuf ex_a, ex_b, ex_r;     // explicit stepping: stage states and residual
   SolverState* solver = nullptr;   // Newton-Krylov work space (sg_solver.cpp)
   size_t ncell() const { return (size_t)geo.nic * geo.njc; }
   size_t nsoa() const { return (size_t)nvar * g.plane; }

@@ -1,5 +1,7 @@
 // Plain-data descriptors shared by the host context and the kernels.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstddef>
 
 namespace sg {
@@ -28,7 +30,12 @@ struct DevPhys {
   double dpdx, dpdy;
   int order;                 // 1 or 2
   int viscous;               // mu_inf > 1e-15
+  int flux;                  // FluxKind
+  int pad_;
 };
+
+// Convective flux scheme (solver.flux).
+enum FluxKind { FLUX_ROE = 0, FLUX_AUSM = 1 };
 
 // One boundary-condition evaluation.  Operand refs: >= 0 is an interior
 // cell (cj*nic + ci), < 0 is another node -(ref+1).
@@ -70,5 +77,17 @@ constexpr double kGamma = 1.4;
 constexpr double kCb1 = 0.1355, kSigma = 2.0 / 3.0, kCb2 = 0.622, kKappa = 0.41;
 constexpr double kCw2 = 0.3, kCw3 = 2.0, kCv1 = 7.1, kCv2 = 0.7, kCv3 = 0.9, kPrt = 0.9;
 constexpr double kCt3 = 1.2, kCn1 = 16.0;
+
+// Local time step of one cell, EulerEquation::calc_dt
+// (src/model/eulerequation.cpp:237-259): CFL / (lambda/l + 2 mu_inf/l^2) with
+// lambda = c + |u| + |v| and l the shorter face length; IEEE divisions and
+// square root in the reference's order, so the value is the reference's.
+__device__ __forceinline__ double cell_dt(double r, double ru, double rv, double rE, double len, double mu_inf,
+                                          double cfl) {
+  const double u = ru / r, v = rv / r;
+  const double p = (rE - 0.5 * r * (u * u + v * v)) * (kGamma - 1.0);
+  const double lam = sqrt(kGamma * p / r) + fabs(u) + fabs(v);
+  return cfl / (lam / len + 2.0 * mu_inf / len / len);
+}
 
 }  // namespace sg

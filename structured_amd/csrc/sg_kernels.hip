@@ -97,11 +97,7 @@ __global__ void dt_kernel(DevGeom g, double mu_inf, int nv, const double* q, con
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= g.nic) return;
   const size_t c = (size_t)j * g.ldi + i;
-  double r = q[c], u = q[g.plane + c] / r, v = q[2 * g.plane + c] / r, rE = q[3 * g.plane + c];
-  double p = (rE - 0.5 * r * (u * u + v * v)) * (kGamma - 1.0);
-  double lam = sqrt(kGamma * p / r) + fabs(u) + fabs(v);
-  double len = dsmin[c];
-  double val = cfl / (lam / len + 2.0 * mu_inf / len / len);
+  const double val = cell_dt(q[c], q[g.plane + c], q[2 * g.plane + c], q[3 * g.plane + c], dsmin[c], mu_inf, cfl);
   for (int k = 0; k < nv; ++k) dt[k * g.plane + c] = val;
 }
 

@@ -204,6 +204,55 @@ __device__ __forceinline__ void roe_flux(double nx, double ny, S rl, S ul, S vl,
   if (SA) f[4] = 0.5 * (f[0] * (ntl + ntr) - sabs(f[0]) * (ntr - ntl));
 }
 
+// AUSM flux (Liou-Steffen), src/model/flux.cpp:149-224: split Mach numbers
+// M+-(M) = +-(M +- 1)^2 / 4 and pressures P+-(M) = p (M +- 1)^2 (2 -+ M) / 4
+// for |M| <= 1, otherwise the upwind values (M+ = M, P+ = p for M > 1 and 0
+// for M < -1; mirrored for M-P-); the interface Mach number picks the side
+// whose rho a (1, u, v, h) is convected.  With SA, rho*nu~ is carried
+// first-order upwind by the AUSM mass flux, as for Roe.
+template <class S, bool SA>
+__device__ __forceinline__ void ausm_flux(double nx, double ny, S rl, S ul, S vl, S pl, S rr, S ur, S vr, S pr, S ntl,
+                                          S ntr, S* f) {
+  const double ds = fsqrt(nx * nx + ny * ny), ids = frcp(ds), gg = kGamma / (kGamma - 1.0);
+  S rli = srcp(rl), rri = srcp(rr);
+  S al = ssqrt(kGamma * pl * rli), ar = ssqrt(kGamma * pr * rri);
+  S Ml = (ul * nx + vl * ny) * ids * srcp(al), Mr = (ur * nx + vr * ny) * ids * srcp(ar);
+  S mp, pp, mm, pm;
+  if (fabs(val(Ml)) <= 1.0) {
+    S a = Ml + 1.0, a2 = 0.25 * a * a;
+    mp = a2; pp = pl * a2 * (2.0 - Ml);
+  } else if (val(Ml) > 0.0) {
+    mp = Ml; pp = pl;
+  } else {
+    mp = S(0.0); pp = S(0.0);
+  }
+  if (fabs(val(Mr)) <= 1.0) {
+    S b = Mr - 1.0, b2 = 0.25 * b * b;
+    mm = -b2; pm = pr * b2 * (2.0 + Mr);
+  } else if (val(Mr) < 0.0) {
+    mm = Mr; pm = pr;
+  } else {
+    mm = S(0.0); pm = S(0.0);
+  }
+  S mh = mp + mm, ph = pp + pm;
+  const bool up = val(mh) >= 0.0;
+  S r = up ? rl : rr, a = up ? al : ar, u = up ? ul : ur, v = up ? vl : vr;
+  S h = gg * (up ? pl * rli : pr * rri) + 0.5 * (u * u + v * v);
+  S m = r * a * mh * ds;
+  f[0] = m;
+  f[1] = m * u + ph * nx;
+  f[2] = m * v + ph * ny;
+  f[3] = m * h;
+  if (SA) f[4] = 0.5 * (f[0] * (ntl + ntr) - sabs(f[0]) * (ntr - ntl));
+}
+
+template <class S, bool SA>
+__device__ __forceinline__ void conv_flux(int kind, double nx, double ny, S rl, S ul, S vl, S pl, S rr, S ur, S vr,
+                                          S pr, S ntl, S ntr, S* f) {
+  if (kind == FLUX_AUSM) ausm_flux<S, SA>(nx, ny, rl, ul, vl, pl, rr, ur, vr, pr, ntl, ntr, f);
+  else roe_flux<S, SA>(nx, ny, rl, ul, vl, pl, rr, ur, vr, pr, ntl, ntr, f);
+}
+
 template <class S>
 struct FaceGeo {
   double tx, ty, bx, by, lx, ly, rx, ry, ivol;  // top/bottom/left/right normals, 1/auxiliary volume
@@ -384,8 +433,8 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
       S ntl(0.0), ntr(0.0);
       if (SA) { ntl = at(L_N, xl, y); ntr = at(L_N, xr, y); }
       const double nx = ncx(fi, cj), ny = ncy(fi, cj);
-      roe_flux<S, SA>(nx, ny, st[0][0], st[0][1], st[0][2], st[0][3], st[1][0], st[1][1], st[1][2], st[1][3], ntl, ntr,
-                      hc);
+      conv_flux<S, SA>(ph.flux, nx, ny, st[0][0], st[0][1], st[0][2], st[0][3], st[1][0], st[1][1], st[1][2], st[1][3],
+                       ntl, ntr, hc);
       if (ph.viscous) {
         FaceGeo<S> G;
         if (fi == 0) {
@@ -446,8 +495,8 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
       S nts(0.0), ntn(0.0);
       if (SA) { nts = at(L_N, x, ys); ntn = at(L_N, x, yn); }
       const double nx = nex(ci, fj), ny = ney(ci, fj);
-      roe_flux<S, SA>(nx, ny, st[0][0], st[0][1], st[0][2], st[0][3], st[1][0], st[1][1], st[1][2], st[1][3], nts, ntn,
-                      he);
+      conv_flux<S, SA>(ph.flux, nx, ny, st[0][0], st[0][1], st[0][2], st[0][3], st[1][0], st[1][1], st[1][2], st[1][3],
+                       nts, ntn, he);
       if (ph.viscous) {
         FaceGeo<S> G;
         if (fj == 0) {

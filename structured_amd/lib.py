@@ -84,8 +84,8 @@ class Equation:
         yv = _f64(case.yv)
         fs = abi.Freestream(**case.fs)
         bcs = abi.make_bcs(case.bcs)
-        o = abi.Opts(order=case.order, turbulence=case.turbulence, tail=case.tail, reserved=0, dpdx=case.dpdx,
-                     dpdy=case.dpdy)
+        o = abi.Opts(order=case.order, turbulence=case.turbulence, tail=case.tail, flux=abi.FLUXES[case.flux],
+                     dpdx=case.dpdx, dpdy=case.dpdy)
         h = C.c_void_p()
         rc = lib.sg_create(case.ni, case.nj, _p(xv), _p(yv), C.byref(fs), C.cast(bcs, C.c_void_p), len(case.bcs),
                            C.byref(o), C.byref(h))
@@ -215,6 +215,27 @@ class Equation:
         report["l2_final"] = list(rep.l2_final)[: self.nvar]
         report["l2w_final"] = list(rep.l2w_final)[: self.nvar]
         return q, report, {"l2": l2h[:n], "cfl": cfl[:n]}
+
+    def step_explicit_dev(self, q_ptr: int, scheme="rk4_jameson", cfl=1.0):
+        """One explicit step on a device SoA state in place; returns the
+        last stage's residual norms."""
+        l2 = np.zeros(self.nvar)
+        sc = abi.SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        self._check(load().sg_step_explicit_dev(self.h, C.c_void_p(q_ptr), sc, C.c_double(cfl), _p(l2)))
+        return l2
+
+    def solve_explicit(self, q, **opts):
+        """Solver::solve with forward Euler or RK4-Jameson from q; returns
+        (q, report dict, per-step norms [steps, nvar])."""
+        q = _f64(q, self.shape).copy()
+        o = abi.explicit_opts(**opts)
+        hist = np.zeros((o.iteration_max + 2, self.nvar))
+        rep = abi.SolveReport()
+        self._check(load().sg_solve_explicit(self.h, _p(q), C.byref(o), C.byref(rep), _p(hist)))
+        report = {f: getattr(rep, f) for f, _ in abi.SolveReport._fields_}
+        report["l2_initial"] = list(rep.l2_initial)[: self.nvar]
+        report["l2_final"] = list(rep.l2_final)[: self.nvar]
+        return q, report, hist[: rep.iterations]
 
     def ls_preallocate(self, restart=30, precond=2, maxit=60, rtol=1e-3):
         self._check(load().sg_ls_preallocate(self.h, restart, precond, maxit, C.c_double(rtol)))

@@ -10,7 +10,10 @@ list, options, state, J.v direction) and the reference's outputs:
   surface  write_surface text (x, Cp, Cf)      (src/utils/io.cpp:181)
   jv_fd    central difference of the reference residual along v
            (h = 1e-6 * |q|-scaled), a derivative check of the tangent.
-Run:  python tests/golden/make_golden.py
+explicit_*.npz hold the reference's Solver::solve result for the explicit
+schemes (forward Euler, RK4-Jameson) from a fixture's state (explicit.json).
+Run:  python tests/golden/make_golden.py [case names]
+      python tests/golden/make_golden.py explicit
 """
 import json
 import os
@@ -51,6 +54,29 @@ def bump_iso_case():
     return cases.Case("bump_iso_33x17", xv, yv, fs, bcs, order=2, turbulence=0, tail=1)
 
 
+def ausm_plate_case():
+    c = cases.laminar_plate(25, 17)
+    c.flux = "ausm"
+    c.name = "laminar_plate_25x17_ausm"
+    return c
+
+
+def ausm_supersonic_case():
+    """Inviscid Mach 1.6 bump channel: AUSM faces on both branches of the
+    split Mach numbers (|M| > 1 along i, |M| < 1 along j) and both signs of
+    the interface Mach number."""
+    xv, yv = grids.bump_channel(33, 17, 1e-2, 0.04, 0.2)
+    bcs = [dict(type="slipwall", face="bottom", start=1, end=-2),
+           dict(type="slipwall", face="top", start=1, end=-2),
+           dict(type="freestream", face="left", start=0, end=-1),
+           dict(type="outflow", face="right", start=0, end=-1)]
+    fs = dict(rho_inf=1.0, u_inf=1.6, v_inf=0.0, p_inf=1 / 1.4, T_inf=1 / 1.4, mu_inf=0.0, pr_inf=0.72,
+              aoa_deg=0.0, nut_inf=0.0)
+    c = cases.Case("inviscid_bump_ausm_33x17", xv, yv, fs, bcs, order=2, turbulence=0, tail=1)
+    c.flux = "ausm"
+    return c
+
+
 def all_cases():
     out = [cases.channel_example(), cases.laminar_plate(25, 17), wake_case(), bump_iso_case()]
     fo = cases.laminar_plate(21, 13)
@@ -61,14 +87,22 @@ def all_cases():
     inv.fs = dict(inv.fs, mu_inf=0.0)
     inv.name = "inviscid_plate_21x13"
     out.append(inv)
+    out += [ausm_plate_case(), ausm_supersonic_case()]
     return out
 
 
-def main():
+def main(only=None):
+    """Writes every fixture, or only those named in `only` (the other entries
+    of index.json are kept)."""
     if not of.have_ref():
         sys.exit("reference harness missing: make -C oracle ref")
     index = {}
+    if only:
+        with open(os.path.join(HERE, "index.json")) as f:
+            index = json.load(f)
     for case in all_cases():
+        if only and case.name not in only:
+            continue
         q = case.perturbed_state(seed=7)
         rng = np.random.default_rng(11)
         v = rng.standard_normal(q.shape) * np.abs(q).max(axis=(0, 1))
@@ -85,10 +119,48 @@ def main():
                             gT_e=grads[5])
         index[case.name] = dict(fs=case.fs, bcs=case.bcs, order=case.order, turbulence=case.turbulence,
                                 tail=case.tail, dpdx=case.dpdx, dpdy=case.dpdy, cfl=3.0, fd_h=h)
+        if case.flux != "roe":
+            index[case.name]["flux"] = case.flux
         print(case.name, "max|R| %.3e" % np.abs(R).max())
     with open(os.path.join(HERE, "index.json"), "w") as f:
         json.dump(index, f, indent=1, sort_keys=True)
 
 
+# Explicit pseudo-time runs of the reference's own Solver::solve
+# (src/solver/solver.cpp:103-121,197-236) from a fixture's state: name ->
+# (base fixture, [solver] keys).  The channel run uses the reference's
+# examples/laminar/channel/explicit.inp settings with fewer iterations.
+EXPLICIT = {
+    "explicit_rk4_channel_3x101": ("channel_3x101", dict(scheme="rk4_jameson", cfl=0.1, iteration_max=20)),
+    "explicit_fe_ramp_laminar_plate_25x17": ("laminar_plate_25x17",
+                                             dict(scheme="forward_euler", cfl=0.8, iteration_max=3, cfl_ramp=True,
+                                                  cfl_ramp_iteration=1, cfl_ramp_exponent=1.2)),
+    "explicit_rk4_laminar_plate_25x17_ausm": ("laminar_plate_25x17_ausm",
+                                              dict(scheme="rk4_jameson", cfl=1.5, iteration_max=3)),
+}
+
+
+def explicit_fixtures():
+    """Writes explicit.json and explicit_*.npz (q0 = the base fixture's
+    state, q = the reference's state after Solver::solve)."""
+    import golden_util as gu
+    index = {}
+    for name, (base, solver) in EXPLICIT.items():
+        case, z, _ = gu.load(base)
+        q = RefRunSolve(case, z["q"], solver)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), q=q)
+        index[name] = dict(base=base, solver=solver)
+        print(name, "max|dq| %.3e" % np.abs(q - z["q"]).max())
+    with open(os.path.join(HERE, "explicit.json"), "w") as f:
+        json.dump(index, f, indent=1, sort_keys=True)
+
+
+def RefRunSolve(case, q, solver):
+    return of.RefRun(case).solve(q, **solver)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["explicit"]:
+        explicit_fixtures()
+    else:
+        main(sys.argv[1:])

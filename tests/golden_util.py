@@ -19,5 +19,20 @@ def load(name):
         meta = json.load(f)[name]
     z = np.load(os.path.join(HERE, name + ".npz"))
     case = cases.Case(name, z["xv"], z["yv"], meta["fs"], meta["bcs"], order=meta["order"],
-                      turbulence=meta["turbulence"], tail=meta["tail"], dpdx=meta["dpdx"], dpdy=meta["dpdy"])
+                      turbulence=meta["turbulence"], tail=meta["tail"], dpdx=meta["dpdx"], dpdy=meta["dpdy"],
+                      flux=meta.get("flux", "roe"))
     return case, {k: z[k] for k in z.files}, meta
+
+
+def explicit_names():
+    with open(os.path.join(HERE, "explicit.json")) as f:
+        return sorted(json.load(f).keys())
+
+
+def load_explicit(name):
+    """(case, q0, solver keys, the reference's final q) of an explicit run."""
+    with open(os.path.join(HERE, "explicit.json")) as f:
+        meta = json.load(f)[name]
+    case, z, _ = load(meta["base"])
+    q = np.load(os.path.join(HERE, name + ".npz"))["q"]
+    return case, z["q"], meta["solver"], q

@@ -33,8 +33,8 @@ def _common(case):
     yv = np.ascontiguousarray(case.yv, dtype=np.float64)
     fs = abi.Freestream(**case.fs)
     bcs = abi.make_bcs(case.bcs)
-    o = abi.Opts(order=case.order, turbulence=case.turbulence, tail=case.tail, reserved=0, dpdx=case.dpdx,
-                 dpdy=case.dpdy)
+    o = abi.Opts(order=case.order, turbulence=case.turbulence, tail=case.tail, flux=abi.FLUXES[case.flux],
+                 dpdx=case.dpdx, dpdy=case.dpdy)
     return xv, yv, fs, bcs, o
 
 
@@ -117,6 +117,38 @@ def time_residual(case, q, warm=3, reps=20, threads=0):
                                   C.byref(o), abi.dptr(q), warm, reps, threads)
 
 
+def explicit_solve(case, q, scheme="forward_euler", cfl=1.0, iteration_max=1, cfl_ramp=False, cfl_ramp_iteration=20,
+                   cfl_ramp_exponent=1.1, wd=None, beta=None):
+    """Solver::solve with an explicit scheme, composed from the oracle's
+    residual and dt: steps counter = 0.. each update q (forward Euler
+    q + R dt, or RK4-Jameson stages q + R(q_s) dt / (4 - s) from q_s = q,
+    src/solver/solver.cpp:12-26,103-121), the loop ending after the step
+    whose counter exceeds iteration_max (:136) and the CFL ramping as
+    min(CFL^e, 1e12) once counter > cfl_ramp_iteration (:209-218).
+    Returns (q, per-step norms of the last stage's residual)."""
+    q = np.array(q, dtype=np.float64)
+    hist = []
+    counter = 0
+    while True:
+        step = dt(case, q, cfl)
+        if scheme == "forward_euler":
+            r, l2 = residual(case, q, wd=wd, beta=beta)
+            q = q + r * step
+        else:
+            qs = q
+            for s in range(4):
+                r, l2 = residual(case, qs, wd=wd, beta=beta)
+                qs = q + (r * step) / (4.0 - s)
+            q = qs
+        hist.append(l2)
+        if counter > iteration_max:
+            break
+        counter += 1
+        if cfl_ramp and counter > cfl_ramp_iteration:
+            cfl = min(cfl ** cfl_ramp_exponent, 1e12)
+    return q, np.array(hist)
+
+
 # --------------------------------------------------------------- reference --
 def have_ref():
     return os.path.exists(REF_HARNESS)
@@ -152,6 +184,15 @@ class RefRun:
     def surface(self, q):
         self._run("surf", self._state(q))
         return np.loadtxt(os.path.join(self.dir, "case.surface"))
+
+    def solve(self, q, **solver):
+        """Solver::solve from q with the given [solver] keys (explicit
+        schemes: the reference build has no ADOL-C); returns the final q."""
+        with open(os.path.join(self.dir, "solve.inp"), "w") as f:
+            f.write(cases.to_toml(self.case, "grid.p3d", label="solve", solver=solver))
+        subprocess.run([REF_HARNESS, "solve.inp", "solve", self._state(q), "qout.bin"], cwd=self.dir, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+        return np.fromfile(os.path.join(self.dir, "qout.bin")).reshape(q.shape)
 
     def grads(self, q):
         self._run("grad", self._state(q), "grad.bin")

@@ -18,13 +18,13 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-def _create(case, bcs=None):
+def _create(case, bcs=None, flux=0):
     so = lib.load()
     xv = np.ascontiguousarray(case.xv)
     yv = np.ascontiguousarray(case.yv)
     fs = abi.Freestream(**case.fs)
     b = abi.make_bcs(case.bcs if bcs is None else bcs)
-    o = abi.Opts(order=case.order, turbulence=case.turbulence, tail=case.tail, reserved=0, dpdx=0.0, dpdy=0.0)
+    o = abi.Opts(order=case.order, turbulence=case.turbulence, tail=case.tail, flux=flux, dpdx=0.0, dpdy=0.0)
     h = C.c_void_p()
     rc = so.sg_create(case.ni, case.nj, xv.ctypes.data_as(C.c_void_p), yv.ctypes.data_as(C.c_void_p), C.byref(fs),
                       C.cast(b, C.c_void_p), len(case.bcs if bcs is None else bcs), C.byref(o), C.byref(h))
@@ -59,6 +59,13 @@ def test_slipwall_on_corner_rejected():
     bcs = [dict(type="slipwall", face="bottom", start=0, end=5)] + case.bcs[1:]
     rc, msg = _create(case, bcs)
     assert rc == -2
+
+
+def test_unknown_flux_rejected():
+    """solver.flux other than roe / ausm (the reference logs "Flux not
+    found.", src/model/eulerequation.cpp:126-127) is refused at creation."""
+    rc, msg = _create(cases.laminar_plate(21, 13), flux=7)
+    assert rc == -1 and "flux" in msg
 
 
 def test_bad_shape_rejected():

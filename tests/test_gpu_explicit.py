@@ -1,0 +1,80 @@
+"""Explicit pseudo-time stepping on the GPU (sg_step_explicit_dev,
+sg_solve_explicit) against the reference's own Solver::solve results (golden
+fixtures, forward Euler with the CFL ramp and RK4-Jameson, Roe and AUSM) and
+against the oracle's composition of the same steps with SA.
+
+Tolerance: ||q - q_ref||_inf <= 1e-11 ||q_ref - q0||_inf (the residuals agree
+to ~1e-13 of their size; a few steps do not amplify that beyond 1e-11)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import golden_util as gu
+import oracle_ffi as of
+from structured_amd import abi, cases, lib
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", gu.explicit_names())
+def test_solve_explicit_vs_reference(name):
+    case, q0, solver, q_ref = gu.load_explicit(name)
+    eq = lib.Equation(case)
+    q, rep, hist = eq.solve_explicit(q0, **solver)
+    err = np.abs(q - q_ref).max() / np.abs(q_ref - q0).max()
+    print(name, "steps", rep["iterations"], "rel err", err, flush=True)
+    assert rep["iterations"] == solver["iteration_max"] + 2
+    assert err <= 1e-11
+    _, hist_o = of.explicit_solve(case, q0, **solver)
+    np.testing.assert_allclose(hist, hist_o, rtol=1e-11)
+
+
+@pytest.mark.parametrize("scheme", ["forward_euler", "rk4_jameson"])
+def test_sa_explicit_vs_oracle(scheme):
+    case = cases.sa_plate(33, 25, dy1=1e-4)
+    q0 = case.perturbed_state(seed=5)
+    eq = lib.Equation(case)
+    wd = of.wall_distance(case)
+    beta = 1.0 + 0.2 * np.sin(np.arange(case.nic * case.njc).reshape(case.nic, case.njc))
+    eq.set_beta(beta)
+    solver = dict(scheme=scheme, cfl=2.0, iteration_max=2, cfl_ramp=True, cfl_ramp_iteration=0,
+                  cfl_ramp_exponent=1.1)
+    q, rep, hist = eq.solve_explicit(q0, **solver)
+    qo, hist_o = of.explicit_solve(case, q0, wd=wd, beta=beta, **solver)
+    err = np.abs(q - qo).max() / np.abs(qo - q0).max()
+    print(scheme, "SA rel err", err, flush=True)
+    assert err <= 1e-11
+    np.testing.assert_allclose(hist, hist_o, rtol=1e-11)
+    # counters 0..3 step, the ramp follows steps 0, 1 and 2
+    assert rep["cfl_final"] == pytest.approx(2.0 ** (1.1 ** 3), rel=1e-13)
+
+
+def test_step_explicit_dev_matches_solve():
+    """Two RK4 steps through the device entry point equal the oracle's two
+    steps (the stage buffers are internal; q is updated in place)."""
+    import torch
+    case = cases.laminar_plate(49, 33)
+    q0 = case.perturbed_state(seed=3)
+    eq = lib.Equation(case)
+    dev = torch.device("cuda:0")
+    aos = torch.from_numpy(np.ascontiguousarray(q0).reshape(-1)).to(dev)
+    qd = torch.zeros(eq.nvar * eq.plane, dtype=torch.float64, device=dev)
+    eq.aos_to_soa_dev(aos.data_ptr(), qd.data_ptr())
+    l2s = [eq.step_explicit_dev(qd.data_ptr(), "rk4_jameson", 0.7) for _ in range(2)]
+    eq.soa_to_aos_dev(qd.data_ptr(), aos.data_ptr())
+    eq.synchronize()
+    q = aos.cpu().numpy().reshape(q0.shape)
+    qo, hist_o = of.explicit_solve(case, q0, scheme="rk4_jameson", cfl=0.7, iteration_max=0)
+    assert np.abs(q - qo).max() <= 1e-11 * np.abs(qo - q0).max()
+    np.testing.assert_allclose(np.array(l2s), hist_o, rtol=1e-11)
+
+
+def test_explicit_bad_arguments():
+    case = cases.laminar_plate(21, 13)
+    eq = lib.Equation(case)
+    q = case.perturbed_state(seed=1)
+    so = lib.load()
+    for o in (abi.explicit_opts(scheme=5), abi.explicit_opts(cfl=-1.0), abi.explicit_opts(iteration_max=-1)):
+        rc = so.sg_solve_explicit(eq.h, q.ctypes.data_as(C.c_void_p), C.byref(o), None, None)
+        assert rc == -1, rc

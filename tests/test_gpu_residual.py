@@ -23,6 +23,10 @@ def sa_cases():
     out.append(c)
     b = cases.sa_bump(41, 21, dy1=1e-3)
     out.append(b)
+    a = cases.sa_bump(41, 21, dy1=1e-3)
+    a.flux = "ausm"
+    a.name += "_ausm"
+    out.append(a)
     return out
 
 
@@ -69,7 +73,7 @@ def test_wall_vs_oracle(name):
     assert np.abs(f - fo).max() <= 1e-12 * np.abs(fo).max()
 
 
-@pytest.mark.parametrize("k", [0, 1])
+@pytest.mark.parametrize("k", [0, 1, 2])
 def test_sa_residual_and_jv_vs_oracle(k):
     case = sa_cases()[k]
     q = case.perturbed_state(seed=5)

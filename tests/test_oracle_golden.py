@@ -50,7 +50,8 @@ def test_jv_dual_matches_reference_fd(name):
 @pytest.mark.skipif(not of.have_ref(), reason="compiled reference (oracle/_ref) not built")
 def test_live_reference_seeds():
     """Fresh seeds against the live compiled reference."""
-    for name in ("laminar_plate_25x17", "wake_25x11", "bump_iso_33x17"):
+    for name in ("laminar_plate_25x17", "wake_25x11", "bump_iso_33x17", "laminar_plate_25x17_ausm",
+                 "inviscid_bump_ausm_33x17"):
         case, _, _ = gu.load(name)
         ref = of.RefRun(case)
         for seed in (3, 4):
@@ -58,3 +59,20 @@ def test_live_reference_seeds():
             r, _ = of.residual(case, q)
             rr = ref.residual(q)
             assert np.abs(r - rr).max() <= 1e-13 * np.abs(rr).max()
+
+
+@pytest.mark.parametrize("name", gu.explicit_names())
+def test_explicit_solve_matches_reference(name):
+    """The oracle's composition of forward Euler / RK4-Jameson steps, the
+    stop rule and the CFL ramp against the reference's own Solver::solve
+    (compiled without ADOL-C) from the same state."""
+    case, q0, solver, q_ref = gu.load_explicit(name)
+    q, hist = of.explicit_solve(case, q0, **solver)
+    assert len(hist) == solver["iteration_max"] + 2
+    assert np.abs(q - q_ref).max() <= 1e-12 * np.abs(q_ref - q0).max()
+    # a dropped stage, a wrong stage weight or an off-by-one step count moves q
+    # by far more than that tolerance
+    for wrong in (dict(solver, iteration_max=solver["iteration_max"] - 1),
+                  dict(solver, scheme="forward_euler" if solver["scheme"] != "forward_euler" else "rk4_jameson")):
+        qw, _ = of.explicit_solve(case, q0, **wrong)
+        assert np.abs(qw - q_ref).max() > 1e-6 * np.abs(q_ref - q0).max()
