@@ -155,6 +155,27 @@ int sg_dt(sg_ctx* ctx, const double* q_aos, double cfl, double* dt_aos);
  *      wall Cf uses the eta-face gradients of that evaluation. */
 int sg_wall(sg_ctx* ctx, const double* q_aos, double* x, double* cp, double* cf, double* forces);
 
+/* ---- files: the Mesh grid loaders (src/utils/mesh.cpp:133-169) and the
+ *      IOManager writers (src/utils/io.cpp:41-255), same layouts and text
+ *      formatting as the reference ---- */
+/* format "p3d" (plot3d_loader: "1 ni nj", then x, then y, j-outer i-inner) or
+ * "simple" (simple_loader: x y pairs, j-outer i-inner).  xv, yv: [ni][nj].
+ * A header that does not match ni, nj or a short file is SG_ERR_ARG (the
+ * reference asserts); errors read back through sg_last_error(NULL). */
+int sg_read_grid(const char* path, const char* format, int ni, int nj, double* xv, double* yv);
+/* <label>.out: raw float64 q[i][j][k] (write_restart, io.cpp:134-152) */
+int sg_write_restart(sg_ctx* ctx, const char* path, const double* q_aos);
+/* read_restart (io.cpp:154-179): a file of any other size than
+ * nic*njc*nvar doubles is SG_ERR_ARG (the reference asserts) */
+int sg_read_restart(sg_ctx* ctx, const char* path, double* q_aos);
+/* <label>.surface: "x cp cf" per wall cell (write_surface, io.cpp:181-255),
+ * from sg_wall at q; forces (optional, 6 entries) as sg_wall */
+int sg_write_surface(sg_ctx* ctx, const char* path, const double* q_aos, double* forces);
+/* <label>.tec: Tecplot point file of xc, yc, q (+ psi_0 for SA) and the ten
+ * q_aux columns (write_tecplot, io.cpp:49-99; the reference never fills
+ * q_aux, written here as 0) */
+int sg_write_tecplot(sg_ctx* ctx, const char* path, const double* q_aos);
+
 /* ---- device-resident variants (SoA device pointers, see header comment) */
 int sg_residual_dev(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_host);
 int sg_jv_dev(sg_ctx* ctx, const double* q_dev, const double* v_dev, double* jv_dev);

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libstructured_gpu.so")
 SOURCES = ["sg_api.cpp", "sg_kernels.hip", "sg_linalg.hip", "sg_adjoint.hip", "sg_solver.cpp", "sg_mlp.hip",
-           "sg_mlp_api.cpp", "sg_inversion.cpp", "sg_cilu.hip", "sg_explicit.hip"]
+           "sg_mlp_api.cpp", "sg_inversion.cpp", "sg_cilu.hip", "sg_explicit.hip",
+           "sg_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-fopenmp"]

@@ -55,6 +55,17 @@ def load():
     return _lib
 
 
+def read_grid(path, fmt, ni, nj):
+    """Mesh::simple_loader / plot3d_loader (src/utils/mesh.cpp:133-169):
+    returns xv, yv [ni, nj]."""
+    lib = load()
+    xv, yv = np.empty((ni, nj)), np.empty((ni, nj))
+    rc = lib.sg_read_grid(os.fsencode(path), fmt.encode(), ni, nj, _p(xv), _p(yv))
+    if rc != 0:
+        raise SgError(rc, lib.sg_last_error(None).decode())
+    return xv, yv
+
+
 def exported_symbols():
     """Function names declared in include/structured_gpu.h."""
     import re
@@ -151,6 +162,42 @@ class Equation:
         x, cp, cf, f = np.empty(nb), np.empty(nb), np.empty(nb), np.empty(6)
         self._check(load().sg_wall(self.h, _p(q), _p(x), _p(cp), _p(cf), _p(f)))
         return x, cp, cf, f
+
+    # ---- files (src/utils/io.cpp writers, same layouts)
+    def write_restart(self, path, q):
+        q = _f64(q, self.shape)
+        self._check(load().sg_write_restart(self.h, os.fsencode(path), _p(q)))
+
+    def read_restart(self, path):
+        q = np.empty(self.shape)
+        self._check(load().sg_read_restart(self.h, os.fsencode(path), _p(q)))
+        return q
+
+    def write_surface(self, path, q):
+        q = _f64(q, self.shape)
+        f = np.empty(6)
+        self._check(load().sg_write_surface(self.h, os.fsencode(path), _p(q), _p(f)))
+        return f
+
+    def write_tecplot(self, path, q):
+        q = _f64(q, self.shape)
+        self._check(load().sg_write_tecplot(self.h, os.fsencode(path), _p(q)))
+
+    def write_npz(self, path, q):
+        """<label>.npz of IOManager::write_npz (src/utils/io.cpp:103-126):
+        xc, yc [nic, njc], q [nic, njc, nvar] and the primitives rho, u, v,
+        p, T [nic, njc].  The reference stores its ghosted primitive arrays
+        under an unghosted shape (their leading nic*njc entries); the cell
+        values are written here instead."""
+        q = _f64(q, self.shape)
+        xv, yv = self.case.xv, self.case.yv
+        xc = 0.25 * (xv[:-1, :-1] + xv[1:, :-1] + xv[:-1, 1:] + xv[1:, 1:])
+        yc = 0.25 * (yv[:-1, :-1] + yv[1:, :-1] + yv[:-1, 1:] + yv[1:, 1:])
+        r = q[..., 0]
+        u, v = q[..., 1] / r, q[..., 2] / r
+        p = (q[..., 3] - 0.5 * r * (u * u + v * v)) * 0.4
+        T = p / r / (self.case.fs["p_inf"] / self.case.fs["rho_inf"] / self.case.fs["T_inf"])
+        np.savez(path, xc=xc, yc=yc, q=q, rho=r, u=u, v=v, p=p, T=T)
 
     def set_beta(self, beta):
         beta = _f64(beta, (self.nic, self.njc))

@@ -17,6 +17,7 @@ Run:  python tests/golden/make_golden.py [case names]
 """
 import json
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -147,16 +148,22 @@ def explicit_fixtures():
     index = {}
     for name, (base, solver) in EXPLICIT.items():
         case, z, _ = gu.load(base)
-        q = RefRunSolve(case, z["q"], solver)
+        ref = of.RefRun(case)
+        q = ref.solve(z["q"], **solver)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), q=q)
+        if name in TECPLOT:
+            # the reference's own Tecplot and restart files of the final state
+            # (IOManager::write at the last step, src/solver/solver.cpp:136-143)
+            shutil.copy(os.path.join(ref.dir, "solve.tec"), os.path.join(HERE, name + ".tec"))
+            with open(os.path.join(ref.dir, "solve.out"), "rb") as f:
+                assert np.array_equal(np.frombuffer(f.read(), dtype=np.float64), q.reshape(-1))
         index[name] = dict(base=base, solver=solver)
         print(name, "max|dq| %.3e" % np.abs(q - z["q"]).max())
     with open(os.path.join(HERE, "explicit.json"), "w") as f:
         json.dump(index, f, indent=1, sort_keys=True)
 
 
-def RefRunSolve(case, q, solver):
-    return of.RefRun(case).solve(q, **solver)
+TECPLOT = ("explicit_fe_ramp_laminar_plate_25x17",)
 
 
 if __name__ == "__main__":
