@@ -9,7 +9,8 @@
 // a chunk has njc + 2W - 2 levels of at most W cells.  One work-group per
 // chunk walks the levels with a barrier between them.
 //
-// Factors are stored per chunk in level order, one record per level:
+// Factors are stored in FP32 (the preconditioner only has to approximate
+// A^-1; FGMRES runs in FP64), per chunk in level order, one record per level:
 // rec[((chunk * nlev + L) * NSLOT * B + s * B + e) * W + c] for block entry e
 // of slot s of the cell in column c of the chunk (row j = L - 2c): slot 0 the
 // inverse of the factored diagonal block, lower slots L_rn (unit lower
@@ -60,14 +61,14 @@ __device__ void cinv(const double* A, double* Ainv) {
 
 template <int NV, int W>
 __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int njc, int ldi, int nlev, const double* idt,
-                                                                  const double* blocks, double* rec) {
+                                                                  const double* blocks, float* rec) {
   constexpr int B = NV * NV, RS = NSLOT * B * W;
   __shared__ double A[W][NSLOT][B];
   __shared__ double Lb[W][B];
   const int chunk = blockIdx.x, i0 = chunk * W, wk = min(W, nic - i0);
   const size_t plane = (size_t)njc * ldi;
   const int t = threadIdx.x, c = t / B, e = t % B, r = e / NV, q = e % NV;
-  double* R = rec + (size_t)chunk * nlev * RS;
+  float* R = rec + (size_t)chunk * nlev * RS;
   auto inside = [&](int cc, int jj) { return cc >= 0 && cc < wk && jj >= 0 && jj < njc; };
   for (int L = 0; L < nlev; ++L) {
     const int j = L - 2 * c, i = i0 + c;
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int n
       const int s = kCLow[qi], di = kCDi[s], dj = kCDj[s];
       const int cn = c + di, jn = j + dj;
       const bool nok = valid && inside(cn, jn);
-      const double* Rn = R + (size_t)(nok ? jn + 2 * cn : 0) * RS;
+      const float* Rn = R + (size_t)(nok ? jn + 2 * cn : 0) * RS;
       double l = 0.0;
       if (nok)
         for (int k = 0; k < NV; ++k) l += A[c][s][r * NV + k] * Rn[(size_t)(k * NV + q) * W + cn];
@@ -114,9 +115,9 @@ __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int n
       for (int k = 0; k < B; ++k) Lb[c][k] = Di[k];
     }
     __syncthreads();
-    double* RL = R + (size_t)L * RS;
+    float* RL = R + (size_t)L * RS;
     for (int s = 0; s < NSLOT; ++s)
-      RL[(size_t)(s * B + e) * W + c] = valid ? (s == 0 ? Lb[c][e] : A[c][s][e]) : 0.0;
+      RL[(size_t)(s * B + e) * W + c] = valid ? (float)(s == 0 ? Lb[c][e] : A[c][s][e]) : 0.0f;
     __threadfence_block();
     __syncthreads();
   }
@@ -148,21 +149,21 @@ constexpr int Back[NSLOT] = {0, 2, 1, 4, 3, 8, 7, 6, 5, 10, 9, 12, 11};
 // to an in-range address) and masked afterwards, so the loads of a level can
 // all be in flight together.
 template <int NV, int W, bool TR, bool ASC>
-__device__ __forceinline__ void cilu_fetch(const double* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
+__device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
                                            int k, int c, int L, const double* src, double* f, double* d, double& x) {
   constexpr int B = NV * NV, RS = NSLOT * B * W;
   const int j = L - 2 * c;
   const bool lok = L >= 0 && L < nlev;
   const bool valid = lok && c < wk && j >= 0 && j < njc;
   const int Lc = lok ? L : 0;
-  const double* RL = R + (size_t)Lc * RS;
+  const float* RL = R + (size_t)Lc * RS;
 #pragma unroll
   for (int n = 0; n < 6; ++n) {
     const int s = ASC ? ct::Low[n] : ct::Up[n];
     const int cn = c + ct::Di[s], jn = j + ct::Dj[s];
     const bool nok = valid && cn >= 0 && cn < wk && jn >= 0 && jn < njc;
     if (!TR) {
-      const double* p = RL + (size_t)(s * B + k * NV) * W + c;
+      const float* p = RL + (size_t)(s * B + k * NV) * W + c;
 #pragma unroll
       for (int m = 0; m < NV; ++m) {
         const double v = p[(size_t)m * W];
@@ -170,8 +171,8 @@ __device__ __forceinline__ void cilu_fetch(const double* R, int nlev, int njc, i
       }
     } else {
       // entry of the neighbour's row that points back to this cell
-      const double* Rn = R + (size_t)(nok ? jn + 2 * cn : Lc) * RS;
-      const double* p = Rn + (size_t)(ct::Back[s] * B + k) * W + (nok ? cn : c);
+      const float* Rn = R + (size_t)(nok ? jn + 2 * cn : Lc) * RS;
+      const float* p = Rn + (size_t)(ct::Back[s] * B + k) * W + (nok ? cn : c);
 #pragma unroll
       for (int m = 0; m < NV; ++m) {
         const double v = p[(size_t)m * NV * W];
@@ -230,16 +231,16 @@ __device__ __forceinline__ void cilu_level(int L, int njc, int ldi, size_t plane
 // applying the diagonal inverse when DIAG.  Two operand sets alternate so that
 // the next level's loads stay in flight while the current level computes.
 template <int NV, int W, bool TR, bool ASC, bool DIAG>
-__device__ __forceinline__ void cilu_pass(const double* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
+__device__ __forceinline__ void cilu_pass(const float* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
                                           int k, int c, const double* src, double* z, double (*ring)[8][NV],
                                           double (*tmp)[NV]) {
   double fa[6 * NV], da[NV], xa, fb[6 * NV], db[NV], xb;
   auto lev = [&](int step) { return ASC ? step : nlev - 1 - step; };
   cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(0), src, fa, da, xa);
+  // an odd level count runs one empty level (no valid cell) at the end
   for (int step = 0; step < nlev; step += 2) {
     cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + 1), src, fb, db, xb);
     cilu_level<NV, W, ASC, DIAG>(lev(step), njc, ldi, plane, i0, wk, k, c, fa, da, xa, z, ring, tmp);
-    if (step + 1 >= nlev) break;
     cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + 2), src, fa, da, xa);
     cilu_level<NV, W, ASC, DIAG>(lev(step + 1), njc, ldi, plane, i0, wk, k, c, fb, db, xb, z, ring, tmp);
   }
@@ -248,7 +249,7 @@ __device__ __forceinline__ void cilu_pass(const double* R, int nlev, int njc, in
 // z = (LU)^-1 b, or (LU)^-T b when TR.  Thread (k, c): component k of the
 // cell in column c.  Operands are loaded one level ahead of their use.
 template <int NV, int W, bool TR>
-__global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, int ldi, int nlev, const double* rec,
+__global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, int ldi, int nlev, const float* rec,
                                                             const double* b, double* z) {
   constexpr int B = NV * NV, RS = NSLOT * B * W;
   __shared__ double ring[W][8][NV];
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, in
   const int chunk = blockIdx.x, i0 = chunk * W, wk = min(W, nic - i0);
   const size_t plane = (size_t)njc * ldi;
   const int k = threadIdx.x / W, c = threadIdx.x % W;
-  const double* R = rec + (size_t)chunk * nlev * RS;
+  const float* R = rec + (size_t)chunk * nlev * RS;
   // ring cells never written hold zeros (their f is zero too)
   for (int e = threadIdx.x; e < W * 8 * NV; e += NV * W) (&ring[0][0][0])[e] = 0.0;
   __syncthreads();
@@ -268,13 +269,14 @@ __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, in
   cilu_pass<NV, W, TR, false, !TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, z, z, ring, tmp);
 }
 
+// in doubles of storage (the factors are FP32)
 size_t cilu_size(int nic, int njc, int nv, int w) {
   const int nchunk = (nic + w - 1) / w, nlev = njc + 2 * w - 2;
-  return (size_t)nchunk * nlev * NSLOT * nv * nv * w;
+  return ((size_t)nchunk * nlev * NSLOT * nv * nv * w + 1) / 2;
 }
 
 template <int NV, int W>
-static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const double* blocks, double* rec,
+static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const double* blocks, float* rec,
                           hipStream_t s) {
   const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2;
   hipLaunchKernelGGL((cilu_factor_kernel<NV, W>), dim3(nchunk), dim3(W * NV * NV), 0, s, nic, njc, ldi, nlev, idt,
@@ -282,7 +284,7 @@ static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const do
 }
 
 template <int NV, int W>
-static void cilu_apply_t(int nic, int njc, int ldi, const double* rec, const double* b, double* z, bool tr,
+static void cilu_apply_t(int nic, int njc, int ldi, const float* rec, const double* b, double* z, bool tr,
                          hipStream_t s) {
   const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2;
   if (tr)
@@ -295,8 +297,9 @@ static void cilu_apply_t(int nic, int njc, int ldi, const double* rec, const dou
 
 bool cilu_width_ok(int w) { return w == 8 || w == 16 || w == 32; }
 
-hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, const double* idt, const double* blocks, double* rec,
+hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, const double* idt, const double* blocks, double* store,
                        hipStream_t s) {
+  float* rec = reinterpret_cast<float*>(store);
   if (nv == 5) {
     if (w == 8) cilu_factor_t<5, 8>(nic, njc, ldi, idt, blocks, rec, s);
     else if (w == 16) cilu_factor_t<5, 16>(nic, njc, ldi, idt, blocks, rec, s);
@@ -309,8 +312,9 @@ hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, const double* i
   return hipGetLastError();
 }
 
-hipError_t cilu_apply(int nic, int njc, int ldi, int nv, int w, const double* rec, const double* b, double* z,
+hipError_t cilu_apply(int nic, int njc, int ldi, int nv, int w, const double* store, const double* b, double* z,
                       bool transpose, hipStream_t s) {
+  const float* rec = reinterpret_cast<const float*>(store);
   if (nv == 5) {
     if (w == 8) cilu_apply_t<5, 8>(nic, njc, ldi, rec, b, z, transpose, s);
     else if (w == 16) cilu_apply_t<5, 16>(nic, njc, ldi, rec, b, z, transpose, s);

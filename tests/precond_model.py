@@ -129,9 +129,14 @@ def ilu_mc5(B, idt, v):
     return z
 
 
-def _ilu_ordered(B, idt, v, order, keep, transpose=False):
+def _ilu_ordered(B, idt, v, order, keep, transpose=False, f32=False):
     """Block ILU(0) of I/dt - J on the 13-point stencil with the cells in the
-    given order, keeping only the couplings keep((i,j), (ii,jj)) accepts."""
+    given order, keeping only the couplings keep((i,j), (ii,jj)) accepts.
+    f32: the factors (L, U and the inverted diagonal blocks) are stored
+    rounded to float32 and later rows and the solves use the stored values,
+    as the device factorisation does; a row's own L blocks enter its upper
+    updates unrounded."""
+    rnd = (lambda a: a.astype(np.float32).astype(np.float64)) if f32 else (lambda a: a)
     v = np.asarray(v, dtype=np.float64)
     nic, njc, nv = v.shape
     pos = {c: n for n, c in enumerate(order)}
@@ -147,13 +152,18 @@ def _ilu_ordered(B, idt, v, order, keep, transpose=False):
                 d[pos[(ii, jj)]] = blk
         rows.append(d)
     N = len(order)
+    dinv = [None] * N
     for rr in range(N):
         d = rows[rr]
         for k in sorted(c for c in d if c < rr):
-            d[k] = d[k] @ np.linalg.inv(rows[k][k])
+            d[k] = d[k] @ dinv[k]
             for jc, ukj in rows[k].items():
                 if jc > k and jc in d:
                     d[jc] = d[jc] - d[k] @ ukj
+        dinv[rr] = rnd(np.linalg.inv(d[rr]))
+        for k in d:
+            if k != rr:
+                d[k] = rnd(d[k])
     vv = np.array([v[c] for c in order])
     if not transpose:
         y = np.zeros_like(vv)
@@ -169,7 +179,7 @@ def _ilu_ordered(B, idt, v, order, keep, transpose=False):
             for k, uk in rows[rr].items():
                 if k > rr:
                     s -= uk @ x[k]
-            x[rr] = np.linalg.solve(rows[rr][rr], s)
+            x[rr] = dinv[rr] @ s
     else:
         w = np.zeros_like(vv)
         for rr in range(N):
@@ -177,7 +187,7 @@ def _ilu_ordered(B, idt, v, order, keep, transpose=False):
             for k in rows[rr]:
                 if k < rr:
                     s -= rows[k][rr].T @ w[k]
-            w[rr] = np.linalg.solve(rows[rr][rr].T, s)
+            w[rr] = dinv[rr].T @ s
         x = np.zeros_like(vv)
         for rr in reversed(range(N)):
             s = w[rr].copy()
@@ -198,9 +208,10 @@ def ilu_natural(B, idt, v, transpose=False):
     return _ilu_ordered(B, idt, v, order, lambda a, b: True, transpose)
 
 
-def ilu_chunked(B, idt, v, w, transpose=False):
+def ilu_chunked(B, idt, v, w, transpose=False, f32=True):
     """Chunked line block ILU(0): i-chunks of w columns, each ordered column
-    by column with j fastest; couplings between chunks dropped."""
+    by column with j fastest; couplings between chunks dropped; factors
+    stored in float32 as on the device."""
     nic, njc = np.asarray(v).shape[:2]
     order = [(i, j) for i in range(nic) for j in range(njc)]  # chunks are contiguous in i
-    return _ilu_ordered(B, idt, v, order, lambda a, b: a[0] // w == b[0] // w, transpose)
+    return _ilu_ordered(B, idt, v, order, lambda a, b: a[0] // w == b[0] // w, transpose, f32)

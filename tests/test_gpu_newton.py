@@ -117,7 +117,8 @@ def test_ilu_matches_numpy_model_sa(precond):
 @pytest.mark.parametrize("transpose", [False, True])
 def test_chunked_ilu_matches_numpy_model(width, sa, transpose):
     """Chunked line ILU(0) (precond 6): chunks of `width` columns (a partial
-    last chunk at width 8 on 12 columns), M^-1 and M^-T against the model."""
+    last chunk at width 8 on 12 columns), M^-1 and M^-T against the model
+    with float32 factors."""
     import precond_model as pm
     case = cases.sa_bump(13, 9, dy1=1e-3) if sa else cases.laminar_plate(13, 9, dy1=1e-2)
     q = case.perturbed_state(seed=6, amp=0.01)
@@ -131,9 +132,13 @@ def test_chunked_ilu_matches_numpy_model(width, sa, transpose):
     J = of.jacobian(case, q, wd=eq.walldist() if sa else None)
     B = pm.blocks_from_dense(J, case.nic, case.njc, case.nvar)
     zr = pm.ilu_chunked(B, 1.0 / dt[..., 0], v, width, transpose=transpose)
+    z64 = pm.ilu_chunked(B, 1.0 / dt[..., 0], v, width, transpose=transpose, f32=False)
     err = float(np.abs(z - zr).max() / np.abs(zr).max())
-    print("chunked ILU width", width, "sa", sa, "transpose", transpose, "model err", err)
-    assert err <= 1e-9
+    print("chunked ILU width", width, "sa", sa, "transpose", transpose, "model err", err,
+          "(float64 factors: %.1e)" % float(np.abs(z - z64).max() / np.abs(z64).max()))
+    # the model stores its factors in float32 like the device; rounding at a
+    # float32 tie can differ in the last place, far below 1e-8
+    assert err <= 1e-8
 
 
 def test_first_order_preconditioner_blocks():
