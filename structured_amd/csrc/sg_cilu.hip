@@ -22,107 +22,6 @@
 
 namespace sg {
 
-__constant__ int kCDi[NSLOT] = {0, 0, 0, -1, 1, -1, 1, -1, 1, 0, 0, -2, 2};
-__constant__ int kCDj[NSLOT] = {0, -1, 1, 0, 0, -1, -1, 1, 1, -2, 2, 0, 0};
-// lower / upper slots in increasing chunk order o
-__constant__ int kCLow[6] = {11, 5, 3, 7, 9, 1};
-__constant__ int kCUp[6] = {2, 10, 6, 4, 8, 12};
-
-__device__ __forceinline__ int cslot(int di, int dj) {
-  for (int s = 0; s < NSLOT; ++s)
-    if (kCDi[s] == di && kCDj[s] == dj) return s;
-  return -1;
-}
-
-template <int NV>
-__device__ void cinv(const double* A, double* Ainv) {
-  double M[NV][2 * NV];
-  for (int r = 0; r < NV; ++r)
-    for (int c = 0; c < NV; ++c) { M[r][c] = A[r * NV + c]; M[r][NV + c] = (r == c) ? 1.0 : 0.0; }
-  for (int p = 0; p < NV; ++p) {
-    int piv = p;
-    double best = fabs(M[p][p]);
-    for (int r = p + 1; r < NV; ++r)
-      if (fabs(M[r][p]) > best) { best = fabs(M[r][p]); piv = r; }
-    if (piv != p)
-      for (int c = 0; c < 2 * NV; ++c) { double t = M[p][c]; M[p][c] = M[piv][c]; M[piv][c] = t; }
-    const double ip = 1.0 / M[p][p];
-    for (int c = 0; c < 2 * NV; ++c) M[p][c] *= ip;
-    for (int r = 0; r < NV; ++r) {
-      if (r == p) continue;
-      const double f = M[r][p];
-      if (f != 0.0)
-        for (int c = 0; c < 2 * NV; ++c) M[r][c] -= f * M[p][c];
-    }
-  }
-  for (int r = 0; r < NV; ++r)
-    for (int c = 0; c < NV; ++c) Ainv[r * NV + c] = M[r][NV + c];
-}
-
-template <int NV, int W>
-__global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int njc, int ldi, int nlev, const double* idt,
-                                                                  const double* blocks, float* rec) {
-  constexpr int B = NV * NV, RS = NSLOT * B * W;
-  __shared__ double A[W][NSLOT][B];
-  __shared__ double Lb[W][B];
-  const int chunk = blockIdx.x, i0 = chunk * W, wk = min(W, nic - i0);
-  const size_t plane = (size_t)njc * ldi;
-  const int t = threadIdx.x, c = t / B, e = t % B, r = e / NV, q = e % NV;
-  float* R = rec + (size_t)chunk * nlev * RS;
-  auto inside = [&](int cc, int jj) { return cc >= 0 && cc < wk && jj >= 0 && jj < njc; };
-  for (int L = 0; L < nlev; ++L) {
-    const int j = L - 2 * c, i = i0 + c;
-    const bool valid = inside(c, j);
-    const size_t cell = valid ? (size_t)j * ldi + i : 0;
-    for (int s = 0; s < NSLOT; ++s) {
-      double a = 0.0;
-      if (valid && inside(c + kCDi[s], j + kCDj[s])) {
-        a = -blocks[(size_t)(s * B + e) * plane + cell];
-        if (s == 0 && r == q) a += idt[cell];
-      }
-      A[c][s][e] = a;
-    }
-    __syncthreads();
-    for (int qi = 0; qi < 6; ++qi) {
-      const int s = kCLow[qi], di = kCDi[s], dj = kCDj[s];
-      const int cn = c + di, jn = j + dj;
-      const bool nok = valid && inside(cn, jn);
-      const float* Rn = R + (size_t)(nok ? jn + 2 * cn : 0) * RS;
-      double l = 0.0;
-      if (nok)
-        for (int k = 0; k < NV; ++k) l += A[c][s][r * NV + k] * Rn[(size_t)(k * NV + q) * W + cn];
-      __syncthreads();
-      A[c][s][e] = l;
-      Lb[c][e] = l;
-      __syncthreads();
-      if (nok) {
-        for (int ui = 0; ui < 6; ++ui) {
-          const int ts = kCUp[ui];
-          const int ri = di + kCDi[ts], rj = dj + kCDj[ts];
-          const int su = cslot(ri, rj);
-          if (su < 0 || !inside(c + ri, j + rj)) continue;
-          double sum = 0.0;
-          for (int k = 0; k < NV; ++k) sum += Lb[c][r * NV + k] * Rn[(size_t)(ts * B + k * NV + q) * W + cn];
-          A[c][su][e] -= sum;
-        }
-      }
-      __syncthreads();
-    }
-    if (e == 0 && valid) {
-      double D[B], Di[B];
-      for (int k = 0; k < B; ++k) D[k] = A[c][0][k];
-      cinv<NV>(D, Di);
-      for (int k = 0; k < B; ++k) Lb[c][k] = Di[k];
-    }
-    __syncthreads();
-    float* RL = R + (size_t)L * RS;
-    for (int s = 0; s < NSLOT; ++s)
-      RL[(size_t)(s * B + e) * W + c] = valid ? (float)(s == 0 ? Lb[c][e] : A[c][s][e]) : 0.0f;
-    __threadfence_block();
-    __syncthreads();
-  }
-}
-
 // Work-group barrier that orders LDS only: the level loop keeps the next
 // level's global loads in flight across it (the usual barrier also waits for
 // every outstanding global load).
@@ -141,7 +40,136 @@ constexpr int Dj[NSLOT] = {0, -1, 1, 0, 0, -1, -1, 1, 1, -2, 2, 0, 0};
 constexpr int Low[6] = {11, 5, 3, 7, 9, 1};
 constexpr int Up[6] = {2, 10, 6, 4, 8, 12};
 constexpr int Back[NSLOT] = {0, 2, 1, 4, 3, 8, 7, 6, 5, 10, 9, 12, 11};
+constexpr int slot(int di, int dj) {
+  for (int s = 0; s < NSLOT; ++s)
+    if (Di[s] == di && Dj[s] == dj) return s;
+  return -1;
+}
 }  // namespace ct
+
+// Factorisation, one level after another.  Thread (c, e) holds entry e of
+// every stencil block of its cell's row in registers.  A row's lower
+// neighbours lie on the four previous levels (L-1 .. L-4), whose factored
+// diagonal inverses and upper blocks are kept in an LDS ring, so the level
+// loop reads the 13 Jacobian blocks of its own cells from memory and nothing
+// else; the elimination exchanges one 5x5 block per cell through LDS per
+// lower neighbour.
+template <int NV, int W>
+__global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int njc, int ldi, int nlev, const double* idt,
+                                                                  const double* blocks, float* rec) {
+  constexpr int B = NV * NV, RS = NSLOT * B * W, NR = 4;
+  __shared__ float ring[NR][7][W][B];  // per level: diagonal inverse, then the six upper blocks (ct::Up order)
+  __shared__ double Ab[W][B];
+  __shared__ double Lb[W][B];
+  __shared__ double cb[2][W][NV];
+  const int chunk = blockIdx.x, i0 = chunk * W, wk = min(W, nic - i0);
+  const size_t plane = (size_t)njc * ldi;
+  const int t = threadIdx.x, c = t / B, e = t % B, r = e / NV, q = e % NV;
+  float* R = rec + (size_t)chunk * nlev * RS;
+  auto inside = [&](int cc, int jj) { return cc >= 0 && cc < wk && jj >= 0 && jj < njc; };
+  for (int L = 0; L < nlev; ++L) {
+    const int j = L - 2 * c, i = i0 + c;
+    const bool valid = inside(c, j);
+    const size_t cell = valid ? (size_t)j * ldi + i : 0;
+    double a[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) {
+      double v = 0.0;
+      if (valid && inside(c + ct::Di[s], j + ct::Dj[s])) {
+        v = -blocks[(size_t)(s * B + e) * plane + cell];
+        if (s == 0 && r == q) v += idt[cell];
+      }
+      a[s] = v;
+    }
+#pragma unroll
+    for (int qi = 0; qi < 6; ++qi) {
+      const int s = ct::Low[qi], di = ct::Di[s], dj = ct::Dj[s];
+      const int cn = c + di, jn = j + dj;
+      const bool nok = valid && inside(cn, jn);
+      const int ln = (L + dj + 2 * di) & (NR - 1);  // the neighbour's level in the ring
+      const int cs = nok ? cn : c;
+      Ab[c][e] = a[s];
+      lds_barrier();
+      double l = 0.0;
+      if (nok) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) l += Ab[c][r * NV + k] * (double)ring[ln][0][cs][k * NV + q];
+      }
+      a[s] = l;
+      Lb[c][e] = l;
+      lds_barrier();
+      if (nok) {
+#pragma unroll
+        for (int ui = 0; ui < 6; ++ui) {
+          const int ts = ct::Up[ui];
+          const int ri = di + ct::Di[ts], rj = dj + ct::Dj[ts];
+          const int su = ct::slot(ri, rj);
+          if (su < 0 || !inside(c + ri, j + rj)) continue;
+          double sum = 0.0;
+#pragma unroll
+          for (int k = 0; k < NV; ++k) sum += Lb[c][r * NV + k] * (double)ring[ln][1 + ui][cs][k * NV + q];
+          a[su] -= sum;
+        }
+      }
+    }
+    // the level's L and U blocks are final: store them (record and ring)
+    float* RL = R + (size_t)L * RS;
+    const int lr = L & (NR - 1);
+#pragma unroll
+    for (int s = 1; s < NSLOT; ++s) RL[(size_t)(s * B + e) * W + c] = valid ? (float)a[s] : 0.0f;
+#pragma unroll
+    for (int ui = 0; ui < 6; ++ui) ring[lr][1 + ui][c][e] = valid ? (float)a[ct::Up[ui]] : 0.0f;
+    // inverse of the factored diagonal block: Gauss-Jordan with partial
+    // pivoting on [D | I], column-parallel (thread e < 2 NV of a cell owns
+    // column e; the pivot column is broadcast through LDS each step)
+    Ab[c][e] = a[0];
+    lds_barrier();
+    double col[NV];
+#pragma unroll
+    for (int rr = 0; rr < NV; ++rr) col[rr] = (e < NV) ? Ab[c][rr * NV + e] : ((e - NV == rr) ? 1.0 : 0.0);
+#pragma unroll
+    for (int p = 0; p < NV; ++p) {
+      if (e == p) {
+#pragma unroll
+        for (int rr = 0; rr < NV; ++rr) cb[p & 1][c][rr] = col[rr];
+      }
+      lds_barrier();
+      double cp[NV];
+#pragma unroll
+      for (int rr = 0; rr < NV; ++rr) cp[rr] = cb[p & 1][c][rr];
+      int piv = p;
+      double best = fabs(cp[p]);
+#pragma unroll
+      for (int rr = p + 1; rr < NV; ++rr) {
+        const double v = fabs(cp[rr]);
+        if (v > best) { best = v; piv = rr; }
+      }
+#pragma unroll
+      for (int rr = p + 1; rr < NV; ++rr) {
+        const bool sw = piv == rr;
+        const double x = cp[p], y = cp[rr], u = col[p], w = col[rr];
+        cp[p] = sw ? y : x; cp[rr] = sw ? x : y;
+        col[p] = sw ? w : u; col[rr] = sw ? u : w;
+      }
+      const double ip = 1.0 / cp[p];
+      col[p] *= ip;
+#pragma unroll
+      for (int rr = 0; rr < NV; ++rr) {
+        if (rr == p) continue;
+        const double f = cp[rr];
+        col[rr] = (f != 0.0) ? col[rr] - f * col[p] : col[rr];
+      }
+    }
+    if (e >= NV && e < 2 * NV) {
+#pragma unroll
+      for (int rr = 0; rr < NV; ++rr) Lb[c][rr * NV + e - NV] = col[rr];
+    }
+    lds_barrier();
+    const float dinv = valid ? (float)Lb[c][e] : 0.0f;
+    RL[(size_t)e * W + c] = dinv;
+    ring[lr][0][c][e] = dinv;
+  }
+}
 
 // One level's operands of thread (k, c): f[n*NV + m] for the six neighbours n
 // of the pass, d[m] for the diagonal inverse (row k, or column k when
