@@ -38,7 +38,9 @@ def test_sa_explicit_vs_oracle(scheme):
     wd = of.wall_distance(case)
     beta = 1.0 + 0.2 * np.sin(np.arange(case.nic * case.njc).reshape(case.nic, case.njc))
     eq.set_beta(beta)
-    solver = dict(scheme=scheme, cfl=2.0, iteration_max=2, cfl_ramp=True, cfl_ramp_iteration=0,
+    # CFL 1.2 keeps both schemes stable on this state (RK4-Jameson at 2.0
+    # blows up in the oracle as well)
+    solver = dict(scheme=scheme, cfl=1.2, iteration_max=2, cfl_ramp=True, cfl_ramp_iteration=0,
                   cfl_ramp_exponent=1.1)
     q, rep, hist = eq.solve_explicit(q0, **solver)
     qo, hist_o = of.explicit_solve(case, q0, wd=wd, beta=beta, **solver)
@@ -47,7 +49,7 @@ def test_sa_explicit_vs_oracle(scheme):
     assert err <= 1e-11
     np.testing.assert_allclose(hist, hist_o, rtol=1e-11)
     # counters 0..3 step, the ramp follows steps 0, 1 and 2
-    assert rep["cfl_final"] == pytest.approx(2.0 ** (1.1 ** 3), rel=1e-13)
+    assert rep["cfl_final"] == pytest.approx(1.2 ** (1.1 ** 3), rel=1e-13)
 
 
 def test_step_explicit_dev_matches_solve():
