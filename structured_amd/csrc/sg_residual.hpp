@@ -611,11 +611,12 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
   __syncthreads();
 
   double sq[2 * NV];  // R^2 (reference norm) and (V R)^2 (flux-balance norm)
+  S res[NV];
+  const size_t oidx = (size_t)cj * g.ldi + ci;
 #pragma unroll
   for (int k = 0; k < 2 * NV; ++k) sq[k] = 0.0;
   if (owns) {
     const double V = vol(ci, cj), iV = frcp(V);
-    const size_t cidx = (size_t)cj * g.ldi + ci;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       S hw = lds[k * NT + tid], hE = lds[k * NT + tid + 1];
@@ -626,20 +627,35 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
       if (k == 2) r = r + (-ph.dpdy * V);
       if (SA && k == 4) r = r + V * src;
       r = r * iV;
-      out.put(k * g.plane + cidx, r);
+      res[k] = r;
       sq[k] = val(r) * val(r);
       sq[NV + k] = sq[k] * V * V;
     }
   }
+  // R is stored after the block's norm partials and its arrival are
+  // acknowledged: the wait for those then does not include the block's R
+  // stores, which drain while the last block finalises.
+  auto store_residual = [&]() {
+    if (owns) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) out.put(k * g.plane + oidx, res[k]);
+    }
+  };
+  if (!(partial && nout.l2)) store_residual();
   // ---- block partial sums of R^2 (fixed order: wavefront butterfly, then waves)
   if (partial) {
     const int lane = tid & 63, wv = tid >> 6;
     const int nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
+    // all components step together, so the cross-lane exchanges of the ten
+    // independent sums are in flight at once
 #pragma unroll
-    for (int k = 0; k < 2 * NV; ++k) {
-      double s = sq[k];
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-      if (lane == 0) red[wv][k] = s;
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+      for (int k = 0; k < 2 * NV; ++k) sq[k] += __shfl_xor(sq[k], o);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 2 * NV; ++k) red[wv][k] = sq[k];
     }
     __syncthreads();
     if (tid < 2 * NV) {
@@ -660,6 +676,7 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
       if (tid == 0)
         last = __hip_atomic_fetch_add(nout.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (unsigned)(nb - 1);
       __syncthreads();
+      store_residual();
       if (last) {
         double acc[2 * NV];
 #pragma unroll
@@ -683,10 +700,13 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 2 * NV; ++k) {
-          double s = acc[k];
-          for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-          if (lane == 0) red[wv][k] = s;
+        for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+          for (int k = 0; k < 2 * NV; ++k) acc[k] += __shfl_xor(acc[k], o);
+        }
+        if (lane == 0) {
+#pragma unroll
+          for (int k = 0; k < 2 * NV; ++k) red[wv][k] = acc[k];
         }
         __syncthreads();
         if (tid < 2 * NV) {
