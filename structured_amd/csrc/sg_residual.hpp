@@ -341,6 +341,26 @@ __device__ __forceinline__ SaTerms<S> sa_terms(S r, S n, S ml, const S* gu, cons
   return t;
 }
 
+// Wavefront sum in a fixed order with DPP lane moves (no LDS round trips):
+// xor 1 and xor 2 inside quads, the mirrors inside 8 and 16 lanes, then the
+// row broadcasts of lanes 15 and 31 carry the row sums up; lane 63 ends with
+// the total.  Lanes a move does not write receive 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_lane63(double v) {
+  v += dpp_move<0xB1>(v);        // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);        // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);       // row_half_mirror
+  v += dpp_move<0x140>(v);       // row_mirror
+  v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
 // Fused norm output: with l2 set, the last block finalises the norms.
 struct NormOut {
   unsigned* counter;  // arrival count, 0 between launches
@@ -642,18 +662,13 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
     }
   };
   if (!(partial && nout.l2)) store_residual();
-  // ---- block partial sums of R^2 (fixed order: wavefront butterfly, then waves)
+  // ---- block partial sums of R^2 (fixed order: wavefront DPP sum, then waves)
   if (partial) {
     const int lane = tid & 63, wv = tid >> 6;
     const int nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
-    // all components step together, so the cross-lane exchanges of the ten
-    // independent sums are in flight at once
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-      for (int k = 0; k < 2 * NV; ++k) sq[k] += __shfl_xor(sq[k], o);
-    }
-    if (lane == 0) {
+    for (int k = 0; k < 2 * NV; ++k) sq[k] = wave_sum_lane63(sq[k]);
+    if (lane == 63) {
 #pragma unroll
       for (int k = 0; k < 2 * NV; ++k) red[wv][k] = sq[k];
     }
@@ -700,11 +715,8 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
         }
         __syncthreads();
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-          for (int k = 0; k < 2 * NV; ++k) acc[k] += __shfl_xor(acc[k], o);
-        }
-        if (lane == 0) {
+        for (int k = 0; k < 2 * NV; ++k) acc[k] = wave_sum_lane63(acc[k]);
+        if (lane == 63) {
 #pragma unroll
           for (int k = 0; k < 2 * NV; ++k) red[wv][k] = acc[k];
         }
