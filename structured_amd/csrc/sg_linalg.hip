@@ -5,6 +5,7 @@
 // i-column).  All vectors are SoA with zero padding (length nvar*plane).
 #include <hip/hip_runtime.h>
 
+#include "sg_device.hpp"
 #include "sg_linalg.hpp"
 
 namespace sg {
@@ -12,23 +13,30 @@ namespace sg {
 constexpr int DOT_BLOCK = 256;
 constexpr int DOT_GRID = 512;  // partial sums per dot product
 
-// out_partial[b*k + i] = sum over this block's slice of V_i . w
+// out_partial[b*k + i] = sum over this block's slice of V_i . w, for groups
+// of DOT_KC vectors at a time (accumulators in registers), each summed over
+// the block in a fixed order (DPP wavefront sums, then the four waves).
+constexpr int DOT_KC = 8;
 __global__ __launch_bounds__(DOT_BLOCK) void multi_dot_kernel(const double* V, size_t ldv, int k, const double* w,
                                                               size_t n, double* partial) {
-  __shared__ double red[DOT_BLOCK / 64][32];
+  __shared__ double red[DOT_BLOCK / 64][DOT_KC];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int i0 = 0; i0 < k; i0 += 32) {
-    const int kk = min(32, k - i0);
-    double acc[32];
-    for (int i = 0; i < 32; ++i) acc[i] = 0.0;
+  for (int i0 = 0; i0 < k; i0 += DOT_KC) {
+    const int kk = min(DOT_KC, k - i0);
+    double acc[DOT_KC];
+#pragma unroll
+    for (int i = 0; i < DOT_KC; ++i) acc[i] = 0.0;
     for (size_t e = (size_t)blockIdx.x * DOT_BLOCK + threadIdx.x; e < n; e += (size_t)gridDim.x * DOT_BLOCK) {
       const double we = w[e];
-      for (int i = 0; i < kk; ++i) acc[i] += V[(size_t)(i0 + i) * ldv + e] * we;
+#pragma unroll
+      for (int i = 0; i < DOT_KC; ++i)
+        if (i < kk) acc[i] += V[(size_t)(i0 + i) * ldv + e] * we;
     }
-    for (int i = 0; i < kk; ++i) {
-      double s = acc[i];
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-      if (lane == 0) red[wv][i] = s;
+#pragma unroll
+    for (int i = 0; i < DOT_KC; ++i) acc[i] = wave_sum_lane63(acc[i]);
+    if (lane == 63) {
+#pragma unroll
+      for (int i = 0; i < DOT_KC; ++i) red[wv][i] = acc[i];
     }
     __syncthreads();
     if ((int)threadIdx.x < kk) {
@@ -41,13 +49,14 @@ __global__ __launch_bounds__(DOT_BLOCK) void multi_dot_kernel(const double* V, s
 }
 
 // out[i] = sum_b partial[b*k + i]  (+ optional add into out when accumulate)
-__global__ void dot_finalize_kernel(const double* partial, int nb, int k, double* out, int accumulate) {
+__global__ __launch_bounds__(256) void dot_finalize_kernel(const double* partial, int nb, int k, double* out,
+                                                           int accumulate) {
   const int i = blockIdx.x;
   __shared__ double red[4];
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(size_t)b * k + i];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  s = wave_sum_lane63(s);
+  if ((threadIdx.x & 63) == 63) red[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = red[0] + red[1] + red[2] + red[3];

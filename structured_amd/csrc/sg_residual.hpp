@@ -341,26 +341,6 @@ __device__ __forceinline__ SaTerms<S> sa_terms(S r, S n, S ml, const S* gu, cons
   return t;
 }
 
-// Wavefront sum in a fixed order with DPP lane moves (no LDS round trips):
-// xor 1 and xor 2 inside quads, the mirrors inside 8 and 16 lanes, then the
-// row broadcasts of lanes 15 and 31 carry the row sums up; lane 63 ends with
-// the total.  Lanes a move does not write receive 0.
-template <int CTRL, int ROW_MASK = 0xF>
-__device__ __forceinline__ double dpp_move(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double wave_sum_lane63(double v) {
-  v += dpp_move<0xB1>(v);        // quad_perm [1,0,3,2]
-  v += dpp_move<0x4E>(v);        // quad_perm [2,3,0,1]
-  v += dpp_move<0x141>(v);       // row_half_mirror
-  v += dpp_move<0x140>(v);       // row_mirror
-  v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-  v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
-  return v;
-}
-
 // Fused norm output: with l2 set, the last block finalises the norms.
 struct NormOut {
   unsigned* counter;  // arrival count, 0 between launches
