@@ -191,12 +191,12 @@ __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, in
     const int cn = c + ct::Di[s], jn = j + ct::Dj[s];
     const bool nok = valid && cn >= 0 && cn < wk && jn >= 0 && jn < njc;
     if (!TR) {
+      // the factorisation stores exact zeros for neighbours outside the chunk
+      // or the grid and for invalid cells, so no mask is needed
       const float* p = RL + (size_t)(s * B + k * NV) * W + c;
 #pragma unroll
-      for (int m = 0; m < NV; ++m) {
-        const double v = p[(size_t)m * W];
-        f[n * NV + m] = nok ? v : 0.0;
-      }
+      for (int m = 0; m < NV; ++m) f[n * NV + m] = p[(size_t)m * W];
+      (void)nok;
     } else {
       // entry of the neighbour's row that points back to this cell
       const float* Rn = R + (size_t)(nok ? jn + 2 * cn : Lc) * RS;
@@ -209,10 +209,7 @@ __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, in
     }
   }
 #pragma unroll
-  for (int m = 0; m < NV; ++m) {
-    const double v = RL[(size_t)(TR ? (m * NV + k) : (k * NV + m)) * W + c];
-    d[m] = valid ? v : 0.0;
-  }
+  for (int m = 0; m < NV; ++m) d[m] = RL[(size_t)(TR ? (m * NV + k) : (k * NV + m)) * W + c];
   const double xv = src[k * plane + (valid ? (size_t)j * ldi + i0 + c : 0)];
   x = valid ? xv : 0.0;
 }
@@ -230,9 +227,9 @@ __device__ __forceinline__ void cilu_level(int L, int njc, int ldi, size_t plane
   for (int n = 0; n < 6; ++n) {
     const int s = ASC ? ct::Low[n] : ct::Up[n];
     const int cn = c + ct::Di[s], jn = j + ct::Dj[s];
-    // out-of-chunk / out-of-grid neighbours carry f = 0; read a valid slot
-    const bool nok = cn >= 0 && cn < W && jn >= 0;
-    const double* zr = ring[nok ? cn : c][(nok ? jn : j) & (NR - 1)];
+    // out-of-chunk / out-of-grid neighbours carry f = 0: any in-range ring
+    // entry (all finite) will do
+    const double* zr = ring[min(max(cn, 0), W - 1)][jn & (NR - 1)];
     double t = 0.0;
 #pragma unroll
     for (int m = 0; m < NV; ++m) t += f[n * NV + m] * zr[m];
