@@ -35,19 +35,28 @@ struct Prim {
   S r, u, v, p, T, n;
 };
 
+// Primitives from the conservative values (rho, rho u, rho v, rho E[, rho nu~]).
+template <class S, bool SA>
+__device__ __forceinline__ Prim<S> prim_from_cons(const S* qc, const DevPhys& ph) {
+  Prim<S> P;
+  P.r = qc[0];
+  const S ir = srcp(P.r);
+  P.u = qc[1] * ir;
+  P.v = qc[2] * ir;
+  P.p = (qc[3] - 0.5 * P.r * (P.u * P.u + P.v * P.v)) * (kGamma - 1.0);
+  P.T = P.p * ir * ph.iR;
+  if (SA) P.n = qc[4] * ir;
+  else P.n = S(0.0);
+  return P;
+}
+
 template <class S, bool SA>
 __device__ __forceinline__ Prim<S> prim_interior(const Field<S>& q, const DevGeom& g, const DevPhys& ph, int ci, int cj) {
   const size_t idx = (size_t)cj * g.ldi + ci;
-  Prim<S> P;
-  P.r = q[idx];
-  const S ir = srcp(P.r);
-  P.u = q[g.plane + idx] * ir;
-  P.v = q[2 * g.plane + idx] * ir;
-  P.p = (q[3 * g.plane + idx] - 0.5 * P.r * (P.u * P.u + P.v * P.v)) * (kGamma - 1.0);
-  P.T = P.p * ir * ph.iR;
-  if (SA) P.n = q[4 * g.plane + idx] * ir;
-  else P.n = S(0.0);
-  return P;
+  S qc[5];
+#pragma unroll
+  for (int k = 0; k < (SA ? 5 : 4); ++k) qc[k] = q[k * g.plane + idx];
+  return prim_from_cons<S, SA>(qc, ph);
 }
 
 // src/model/bc.cpp: freestream :25-60, slipwall :77-128, wall :149-204,
@@ -378,8 +387,30 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
   const int nic = g.nic, njc = g.njc;
   const double nan = __builtin_nan("");
 
-  // ---- stage primitives (+ transport properties) of tile + halo in LDS
-  for (int p = tid; p < NPOS; p += NT) {
+  // ---- stage primitives (+ transport properties) of tile + halo in LDS.
+  // A thread stages at most NPASS positions; the conservative values of its
+  // interior positions are loaded first, all in flight together, and only
+  // then converted.
+  constexpr int NPASS = (NPOS + NT - 1) / NT;
+  constexpr int NQ = SA ? 5 : 4;
+  S qraw[NPASS][NQ];
+#pragma unroll
+  for (int pass = 0; pass < NPASS; ++pass) {
+    const int p = tid + pass * NT;
+    const int ci = i0 - 2 + p % RW, cj = j0 - 2 + p / RW;
+    const bool interior = p < NPOS && ci >= 0 && ci < nic && cj >= 0 && cj < njc;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) qraw[pass][k] = S(0.0);
+    if (interior) {
+      const size_t idx = (size_t)cj * g.ldi + ci;
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) qraw[pass][k] = q[k * g.plane + idx];
+    }
+  }
+#pragma unroll
+  for (int pass = 0; pass < NPASS; ++pass) {
+    const int p = tid + pass * NT;
+    if (p >= NPOS) break;
     const int lx = p % RW, ly = p / RW;
     const int ci = i0 - 2 + lx, cj = j0 - 2 + ly;
     S slot[L_MAX];
@@ -388,7 +419,7 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
     int node = -1;
     if (ring) node = bc.ghost_map[ghost_index(ci, cj, nic, njc)];
     if (interior || node >= 0) {
-      Prim<S> P = interior ? prim_interior<S, SA>(q, g, ph, ci, cj) : eval_node<S, SA>(node, bc, q, g, ph);
+      Prim<S> P = interior ? prim_from_cons<S, SA>(qraw[pass], ph) : eval_node<S, SA>(node, bc, q, g, ph);
       slot[L_R] = P.r; slot[L_U] = P.u; slot[L_V] = P.v; slot[L_P] = P.p; slot[L_T] = P.T;
       if (SA) slot[L_N] = P.n;
       const bool box = lx >= 1 && lx < RW - 1 && ly >= 1 && ly < RH - 1;
