@@ -15,7 +15,9 @@
 // of slot s of the cell in column c of the chunk (row j = L - 2c): slot 0 the
 // inverse of the factored diagonal block, lower slots L_rn (unit lower
 // factor), upper slots U_ru.  Cells of a level sit next to each other, so a
-// level's record is read with coalesced loads.
+// level's record is read with coalesced loads.  A second copy holds every
+// block transposed, for M^-T: a column of a block is then read like a row,
+// with the same coalesced loads.
 #include <hip/hip_runtime.h>
 
 #include "sg_linalg.hpp"
@@ -54,10 +56,14 @@ constexpr int slot(int di, int dj) {
 // loop reads the 13 Jacobian blocks of its own cells from memory and nothing
 // else; the elimination exchanges one 5x5 block per cell through LDS per
 // lower neighbour.
+// Record size of one level, in floats.
+template <int NV, int W>
+constexpr int cilu_rs() { return NSLOT * NV * NV * W; }
+
 template <int NV, int W>
 __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int njc, int ldi, int nlev, const double* idt,
-                                                                  const double* blocks, float* rec) {
-  constexpr int B = NV * NV, RS = NSLOT * B * W, NR = 4;
+                                                                  const double* blocks, float* rec, size_t tcopy) {
+  constexpr int B = NV * NV, RS = cilu_rs<NV, W>(), NR = 4;
   __shared__ float ring[NR][7][W][B];  // per level: diagonal inverse, then the six upper blocks (ct::Up order)
   __shared__ double Ab[W][B];
   __shared__ double Lb[W][B];
@@ -66,6 +72,7 @@ __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int n
   const size_t plane = (size_t)njc * ldi;
   const int t = threadIdx.x, c = t / B, e = t % B, r = e / NV, q = e % NV;
   float* R = rec + (size_t)chunk * nlev * RS;
+  float* RT = R + tcopy;
   auto inside = [&](int cc, int jj) { return cc >= 0 && cc < wk && jj >= 0 && jj < njc; };
   for (int L = 0; L < nlev; ++L) {
     const int j = L - 2 * c, i = i0 + c;
@@ -112,11 +119,19 @@ __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int n
         }
       }
     }
-    // the level's L and U blocks are final: store them (record and ring)
+    // the level's L and U blocks are final: store them (record, transposed
+    // record and ring)
     float* RL = R + (size_t)L * RS;
+    float* RTL = RT + (size_t)L * RS;
     const int lr = L & (NR - 1);
+    auto at_n = [&](int s) { return (size_t)(s * B + r * NV + q) * W + c; };
+    auto at_t = [&](int s) { return (size_t)(s * B + q * NV + r) * W + c; };
 #pragma unroll
-    for (int s = 1; s < NSLOT; ++s) RL[(size_t)(s * B + e) * W + c] = valid ? (float)a[s] : 0.0f;
+    for (int s = 1; s < NSLOT; ++s) {
+      const float v = valid ? (float)a[s] : 0.0f;
+      RL[at_n(s)] = v;
+      RTL[at_t(s)] = v;
+    }
 #pragma unroll
     for (int ui = 0; ui < 6; ++ui) ring[lr][1 + ui][c][e] = valid ? (float)a[ct::Up[ui]] : 0.0f;
     // inverse of the factored diagonal block: Gauss-Jordan with partial
@@ -166,52 +181,51 @@ __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int n
     }
     lds_barrier();
     const float dinv = valid ? (float)Lb[c][e] : 0.0f;
-    RL[(size_t)e * W + c] = dinv;
+    RL[at_n(0)] = dinv;
+    RTL[at_t(0)] = dinv;
     ring[lr][0][c][e] = dinv;
   }
 }
 
+// One stored row of a block (NV entries W floats apart), widened to double.
+template <int NV, int W>
+__device__ __forceinline__ void load_row(const float* p, double* out) {
+#pragma unroll
+  for (int m = 0; m < NV; ++m) out[m] = p[(size_t)m * W];
+}
+
 // One level's operands of thread (k, c): f[n*NV + m] for the six neighbours n
 // of the pass, d[m] for the diagonal inverse (row k, or column k when
-// transposed), x the right-hand side.  Every load is unconditional (clamped
-// to an in-range address) and masked afterwards, so the loads of a level can
-// all be in flight together.
+// transposed), x the right-hand side.  R is the record copy of the pass: the
+// plain one for M^-1, the transposed one for M^-T, where a column of a block
+// is a stored row.  The factorisation stores exact zeros for neighbours
+// outside the chunk or the grid and for invalid cells (the transposed pass
+// reads the cell's own slot, zero for an absent neighbour), so nothing is
+// masked and all the loads of a level are in flight together.
 template <int NV, int W, bool TR, bool ASC>
 __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
                                            int k, int c, int L, const double* src, double* f, double* d, double& x) {
-  constexpr int B = NV * NV, RS = NSLOT * B * W;
+  constexpr int RS = cilu_rs<NV, W>();
   const int j = L - 2 * c;
   const bool lok = L >= 0 && L < nlev;
   const bool valid = lok && c < wk && j >= 0 && j < njc;
-  const int Lc = lok ? L : 0;
-  const float* RL = R + (size_t)Lc * RS;
+  const float* RL = R + (size_t)(lok ? L : 0) * RS;
+  auto row = [&](const float* Rlev, int s, int r, int col) { return Rlev + (size_t)(s * NV + r) * NV * W + col; };
 #pragma unroll
   for (int n = 0; n < 6; ++n) {
     const int s = ASC ? ct::Low[n] : ct::Up[n];
-    const int cn = c + ct::Di[s], jn = j + ct::Dj[s];
-    const bool nok = valid && cn >= 0 && cn < wk && jn >= 0 && jn < njc;
     if (!TR) {
-      // the factorisation stores exact zeros for neighbours outside the chunk
-      // or the grid and for invalid cells, so no mask is needed
-      const float* p = RL + (size_t)(s * B + k * NV) * W + c;
-#pragma unroll
-      for (int m = 0; m < NV; ++m) f[n * NV + m] = p[(size_t)m * W];
-      (void)nok;
+      load_row<NV, W>(row(RL, s, k, c), f + n * NV);
     } else {
-      // entry of the neighbour's row that points back to this cell
-      const float* Rn = R + (size_t)(nok ? jn + 2 * cn : Lc) * RS;
-      const float* p = Rn + (size_t)(ct::Back[s] * B + k) * W + (nok ? cn : c);
-#pragma unroll
-      for (int m = 0; m < NV; ++m) {
-        const double v = p[(size_t)m * NV * W];
-        f[n * NV + m] = nok ? v : 0.0;
-      }
+      // column k of the neighbour's block that points back to this cell
+      const int cn = c + ct::Di[s], jn = j + ct::Dj[s];
+      const bool nok = valid && cn >= 0 && cn < wk && jn >= 0 && jn < njc;
+      const float* p = nok ? row(R + (size_t)(jn + 2 * cn) * RS, ct::Back[s], k, cn) : row(RL, s, k, c);
+      load_row<NV, W>(p, f + n * NV);
     }
   }
-#pragma unroll
-  for (int m = 0; m < NV; ++m) d[m] = RL[(size_t)(TR ? (m * NV + k) : (k * NV + m)) * W + c];
-  const double xv = src[k * plane + (valid ? (size_t)j * ldi + i0 + c : 0)];
-  x = valid ? xv : 0.0;
+  load_row<NV, W>(row(RL, 0, k, c), d);
+  x = src[k * plane + (valid ? (size_t)j * ldi + i0 + c : 0)];
 }
 
 // One level of a triangular pass with its operands already in registers.
@@ -275,14 +289,14 @@ __device__ __forceinline__ void cilu_pass(const float* R, int nlev, int njc, int
 // cell in column c.  Operands are loaded one level ahead of their use.
 template <int NV, int W, bool TR>
 __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, int ldi, int nlev, const float* rec,
-                                                            const double* b, double* z) {
-  constexpr int B = NV * NV, RS = NSLOT * B * W;
+                                                            size_t tcopy, const double* b, double* z) {
+  constexpr int RS = cilu_rs<NV, W>();
   __shared__ double ring[W][8][NV];
   __shared__ double tmp[W][NV];
   const int chunk = blockIdx.x, i0 = chunk * W, wk = min(W, nic - i0);
   const size_t plane = (size_t)njc * ldi;
   const int k = threadIdx.x / W, c = threadIdx.x % W;
-  const float* R = rec + (size_t)chunk * nlev * RS;
+  const float* R = rec + (TR ? tcopy : 0) + (size_t)chunk * nlev * RS;
   // ring cells never written hold zeros (their f is zero too)
   for (int e = threadIdx.x; e < W * 8 * NV; e += NV * W) (&ring[0][0][0])[e] = 0.0;
   __syncthreads();
@@ -295,29 +309,34 @@ __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, in
 }
 
 // in doubles of storage (the factors are FP32)
-size_t cilu_size(int nic, int njc, int nv, int w) {
+// floats of one record copy (plain or transposed)
+static size_t cilu_copy_floats(int nic, int njc, int nv, int w) {
   const int nchunk = (nic + w - 1) / w, nlev = njc + 2 * w - 2;
-  return ((size_t)nchunk * nlev * NSLOT * nv * nv * w + 1) / 2;
+  return (size_t)nchunk * nlev * NSLOT * nv * nv * w;
 }
+
+// in doubles of storage: both copies of FP32 factors
+size_t cilu_size(int nic, int njc, int nv, int w) { return cilu_copy_floats(nic, njc, nv, w); }
 
 template <int NV, int W>
 static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const double* blocks, float* rec,
                           hipStream_t s) {
   const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2;
   hipLaunchKernelGGL((cilu_factor_kernel<NV, W>), dim3(nchunk), dim3(W * NV * NV), 0, s, nic, njc, ldi, nlev, idt,
-                     blocks, rec);
+                     blocks, rec, cilu_copy_floats(nic, njc, NV, W));
 }
 
 template <int NV, int W>
 static void cilu_apply_t(int nic, int njc, int ldi, const float* rec, const double* b, double* z, bool tr,
                          hipStream_t s) {
   const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2;
+  const size_t tcopy = cilu_copy_floats(nic, njc, NV, W);
   if (tr)
-    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, true>), dim3(nchunk), dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec, b,
-                       z);
+    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, true>), dim3(nchunk), dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec,
+                       tcopy, b, z);
   else
     hipLaunchKernelGGL((cilu_apply_kernel<NV, W, false>), dim3(nchunk), dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec,
-                       b, z);
+                       tcopy, b, z);
 }
 
 bool cilu_width_ok(int w) { return w == 8 || w == 16 || w == 32; }
