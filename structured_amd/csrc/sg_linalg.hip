@@ -11,7 +11,7 @@
 namespace sg {
 
 constexpr int DOT_BLOCK = 256;
-constexpr int DOT_GRID = 512;  // partial sums per dot product
+constexpr int DOT_GRID = 2048;  // partial sums per dot product (8 blocks per CU)
 
 // out_partial[b*k + i] = sum over this block's slice of V_i . w, for groups
 // of DOT_KC vectors at a time (accumulators in registers), each summed over
@@ -83,7 +83,7 @@ __global__ void multi_axpy_kernel(const double* V, size_t ldv, int k, const doub
 }
 
 hipError_t multi_axpy(const double* V, size_t ldv, int k, const double* c, double* w, size_t n, hipStream_t s) {
-  hipLaunchKernelGGL(multi_axpy_kernel, dim3(1024), dim3(256), 0, s, V, ldv, k, c, w, n);
+  hipLaunchKernelGGL(multi_axpy_kernel, dim3(2048), dim3(256), 0, s, V, ldv, k, c, w, n);
   return hipGetLastError();
 }
 

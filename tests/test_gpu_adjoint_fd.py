@@ -1,5 +1,5 @@
 """Adjoint beta-gradient of the wall objective against central finite
-differences of re-converged solves (SA channel bump, 16 seeded cells).
+differences of re-converged solves (SA channel bump, 10 seeded cells).
 
 F(beta) = sum_i ds_i (Cf_i(q(beta)) - Cf*_i)^2 with R(q(beta), beta) = 0;
 dF/dbeta from one adjoint solve J^T psi = dF/dq and dF/dbeta = -psi^T dR/dbeta.
@@ -38,8 +38,9 @@ def test_adjoint_gradient_matches_central_differences():
     print("F", F, "adjoint", arep, flush=True)
     assert arep["converged"]
     rng = np.random.default_rng(7)
-    # 16 cells in the region the wall Cf is sensitive to (near the wall)
-    cells = [(int(i), int(j)) for i, j in zip(rng.integers(2, case.nic - 2, 16), rng.integers(0, case.njc // 3, 16))]
+    # 10 cells in the region the wall Cf is sensitive to (near the wall): the
+    # first ten of a seeded draw of 16
+    cells = [(int(i), int(j)) for i, j in zip(rng.integers(2, case.nic - 2, 16), rng.integers(0, case.njc // 3, 16))][:10]
     worst = 0.0
     rels = []
 
