@@ -308,7 +308,6 @@ __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, in
   cilu_pass<NV, W, TR, false, !TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, z, z, ring, tmp);
 }
 
-// in doubles of storage (the factors are FP32)
 // floats of one record copy (plain or transposed)
 static size_t cilu_copy_floats(int nic, int njc, int nv, int w) {
   const int nchunk = (nic + w - 1) / w, nlev = njc + 2 * w - 2;

@@ -40,7 +40,9 @@ hipError_t lilu_factor(int nic, int njc, int ldi, int nv, const double* idt, con
                        hipStream_t s);
 hipError_t lilu_apply(int nic, int njc, int ldi, int nv, const double* ilu, const double* b, double* z, bool transpose,
                       hipStream_t s);
-// chunked line block ILU(0) (sg_cilu.hip): chunks of w in {8, 16, 32} columns
+// chunked line block ILU(0) (sg_cilu.hip): chunks of w in {8, 16, 32} columns;
+// cilu_size is the factor storage in doubles (two FP32 copies, plain and
+// transposed), which cilu_factor writes and cilu_apply reads
 bool cilu_width_ok(int w);
 size_t cilu_size(int nic, int njc, int nv, int w);
 hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, const double* idt, const double* blocks, double* rec,
