@@ -172,6 +172,22 @@ def channel_example(path_grid: str | None = None) -> Case:
     return Case("channel_3x101", xv, yv, fs, bcs, order=2, turbulence=0, tail=1, dpdx=-0.02592)
 
 
+def naca0012_example(xv: np.ndarray, yv: np.ndarray) -> Case:
+    """The reference's laminar NACA 0012 example (examples/laminar/naca0012:
+    281x151 C-grid in the "simple" format, Mach 0.5 at 1 degree, mu_inf 1e-4,
+    freestream far field, wake cut on the bottom cells 1-40, wall on 41-240,
+    tail 41), on the grid given (Mesh::simple_loader's xv, yv)."""
+    fs = dict(rho_inf=1.0, u_inf=0.49992384757819563, v_inf=0.0087262032186417558, p_inf=0.7142857142857143,
+              T_inf=0.7142857142857143, mu_inf=1e-4, pr_inf=0.7, aoa_deg=1.0, nut_inf=0.0)
+    bcs = [dict(type="freestream", face="top", start=0, end=-1),
+           dict(type="freestream", face="left", start=0, end=-1),
+           dict(type="freestream", face="right", start=0, end=-1),
+           dict(type="wake", face="bottom", start=1, end=40),
+           dict(type="wall", face="bottom", start=41, end=240, u=0.0, v=0.0)]
+    return Case("naca0012_281x151", np.ascontiguousarray(xv), np.ascontiguousarray(yv), fs, bcs, order=2,
+                turbulence=0, tail=41)
+
+
 def to_toml(case: Case, grid_file: str, label: str = "case", solver: dict | None = None) -> str:
     """Reference-format configuration (src/utils/config.cpp:31-87 keys).
     `solver`: extra [solver] keys (scheme, cfl, iteration_max, cfl_ramp, ...)."""

@@ -12,8 +12,11 @@ list, options, state, J.v direction) and the reference's outputs:
            (h = 1e-6 * |q|-scaled), a derivative check of the tangent.
 explicit_*.npz hold the reference's Solver::solve result for the explicit
 schemes (forward Euler, RK4-Jameson) from a fixture's state (explicit.json).
+naca0012_281x151.npz holds the reference's outputs on its own NACA 0012
+example grid (naca_fixture).
 Run:  python tests/golden/make_golden.py [case names]
       python tests/golden/make_golden.py explicit
+      python tests/golden/make_golden.py naca
 """
 import json
 import os
@@ -166,8 +169,33 @@ def explicit_fixtures():
 TECPLOT = ("explicit_fe_ramp_laminar_plate_25x17",)
 
 
+NACA_GRID = "/root/reference/examples/laminar/naca0012/grid.unf2"
+NACA_SUB = 3  # every third cell in i and j of R and dt is kept
+
+
+def naca_fixture():
+    """naca0012_281x151.npz: the reference's own NACA 0012 example (its grid
+    file and config values, cases.naca0012_example) on a seeded perturbed
+    state (seed 7): the grid, the reference's R and dt at cfl 3 on every
+    third cell, and its surface file."""
+    from structured_amd import lib
+    xv, yv = lib.read_grid(NACA_GRID, "simple", 281, 151)
+    case = cases.naca0012_example(xv, yv)
+    q = case.perturbed_state(seed=7)
+    ref = of.RefRun(case)
+    R = ref.residual(q)
+    dt = ref.dt(q, 3.0)
+    surf = ref.surface(q)
+    np.savez_compressed(os.path.join(HERE, "naca0012_281x151.npz"), xv=xv, yv=yv, seed=7, sub=NACA_SUB,
+                        R_sub=R[::NACA_SUB, ::NACA_SUB], dt_sub=dt[::NACA_SUB, ::NACA_SUB], surface=surf,
+                        l2=np.sqrt((R ** 2).sum(axis=(0, 1))))
+    print("naca0012_281x151 max|R| %.3e" % np.abs(R).max())
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["explicit"]:
         explicit_fixtures()
+    elif sys.argv[1:] == ["naca"]:
+        naca_fixture()
     else:
         main(sys.argv[1:])

@@ -36,3 +36,12 @@ def load_explicit(name):
     case, z, _ = load(meta["base"])
     q = np.load(os.path.join(HERE, name + ".npz"))["q"]
     return case, z["q"], meta["solver"], q
+
+
+def load_naca():
+    """(case, seeded state, fixture arrays) of the reference's NACA 0012
+    example fixture; R_sub / dt_sub hold every sub-th cell."""
+    z = np.load(os.path.join(HERE, "naca0012_281x151.npz"))
+    case = cases.naca0012_example(z["xv"], z["yv"])
+    q = case.perturbed_state(seed=int(z["seed"]))
+    return case, q, {k: z[k] for k in z.files}

@@ -140,3 +140,26 @@ def test_fused_norm_over_many_launches():
         worst = max(worst, float(np.abs(got[k, :nv] - e).max() / e.max()))
     print("fused norm worst rel err over 120 launches: %.2e" % worst)
     assert worst <= 1e-12
+
+
+def test_naca0012_example_vs_reference():
+    """The reference's own NACA 0012 example: residual and norms against the
+    reference's outputs (fixture), J.v and the wall quantities against the
+    oracle, on the full 281x151 C-grid."""
+    case, q, z = gu.load_naca()
+    n = int(z["sub"])
+    eq = lib.Equation(case)
+    r, l2 = eq.calc_residual(q)
+    err = rel(r[::n, ::n], z["R_sub"])
+    print("naca0012 residual rel err", err, flush=True)
+    assert err <= 1e-12
+    np.testing.assert_allclose(l2, z["l2"], rtol=1e-12)
+    np.testing.assert_allclose(eq.calc_dt(q, 3.0)[::n, ::n], z["dt_sub"], rtol=1e-14)
+    v = np.random.default_rng(8).standard_normal(q.shape) * np.abs(q).max(axis=(0, 1))
+    _, jvo = of.jv(case, q, v)
+    assert rel(eq.jv(q, v), jvo) <= 1e-10
+    x, cp, cf, f = eq.wall(q)
+    xo, cpo, cfo, fo = of.surface(case, q)
+    np.testing.assert_array_equal(x, xo)
+    assert np.abs(cf - cfo).max() <= 1e-12 * np.abs(cfo).max()
+    assert np.abs(cp - cpo).max() <= 1e-12 * np.abs(cpo).max()

@@ -76,3 +76,20 @@ def test_explicit_solve_matches_reference(name):
                   dict(solver, scheme="forward_euler" if solver["scheme"] != "forward_euler" else "rk4_jameson")):
         qw, _ = of.explicit_solve(case, q0, **wrong)
         assert np.abs(qw - q_ref).max() > 1e-6 * np.abs(q_ref - q0).max()
+
+
+def test_naca0012_example_matches_reference():
+    """The reference's own NACA 0012 example (C-grid, wake cut, wall, far
+    field; its grid file and config values): residual, norms, dt and the
+    surface file."""
+    case, q, z = gu.load_naca()
+    n = int(z["sub"])
+    r, l2 = of.residual(case, q)
+    assert np.abs(r[::n, ::n] - z["R_sub"]).max() <= 1e-13 * np.abs(z["R_sub"]).max()
+    np.testing.assert_allclose(l2, z["l2"], rtol=1e-13)
+    np.testing.assert_allclose(of.dt(case, q, 3.0)[::n, ::n], z["dt_sub"], rtol=1e-14, atol=0)
+    x, cp, cf, _ = of.surface(case, q)
+    s = z["surface"]
+    np.testing.assert_allclose(x, s[:, 0], rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(cp, s[:, 1], rtol=2e-5, atol=1e-5 * np.abs(s[:, 1]).max())
+    np.testing.assert_allclose(cf, s[:, 2], rtol=2e-5, atol=1e-5 * np.abs(s[:, 2]).max())
