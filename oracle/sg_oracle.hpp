@@ -4,8 +4,8 @@
 // (anandpratap/structured, src/model + src/utils/mesh.cpp), kept in the
 // reference's own pass structure: primitive recovery into ghosted arrays,
 // boundary conditions applied in list order, MUSCL per direction, mu/k over
-// every ghosted cell, face Green-Gauss gradients, face averages, Roe flux,
-// Green-Gauss viscous flux, assembly, division by the cell volume.  It is
+// every ghosted cell, face Green-Gauss gradients, face averages, Roe or AUSM
+// flux, Green-Gauss viscous flux, assembly, division by the cell volume.  It is
 // templated on the scalar so that a forward dual number gives the exact
 // directional derivative J.v of this same discretisation.
 //
@@ -15,10 +15,14 @@
 //
 // Parity status per function (how each is pinned):
 //   metrics, primvars, BCs (freestream/outflow/slipwall/wall/isothermal/
-//   periodic/wake), MUSCL 1st/2nd order, Roe, mu/k, gradients, face averages,
-//   viscous flux, assembly, dt, L2 norm, surface Cp/Cf/forces:
+//   periodic/wake), MUSCL 1st/2nd order, Roe, AUSM, mu/k, gradients, face
+//   averages, viscous flux, assembly, dt, L2 norm, surface Cp/Cf/forces:
 //       pinned against the compiled reference (oracle/_ref) on the same
-//       grids and states, and by closed forms in tests/test_oracle_*.py.
+//       grids and states (golden fixtures, including the reference's own
+//       channel and NACA 0012 examples), and by closed forms in
+//       tests/test_oracle_*.py.  The explicit steps composed from these
+//       (tests/oracle_ffi.py explicit_solve) are pinned against the
+//       reference's own Solver::solve.
 //   Spalart-Allmaras terms, beta field, wall objective, MLP:
 //       the reference has none of these.  Pinned by the published SA-noft2
 //       definition (closed-form checks), central finite differences and

@@ -1,15 +1,15 @@
 // Fused finite-volume residual on LDS tiles (one pass per tile):
 // primitive recovery + ghost boundary cells -> power-law mu/k (+ SA eddy
-// viscosity) -> MUSCL/van-Albada-type reconstruction -> Roe face flux ->
-// Green-Gauss face gradients and viscous flux -> SA source -> assembly /V,
+// viscosity) -> MUSCL/van-Albada-type reconstruction -> Roe or AUSM face flux
+// -> Green-Gauss face gradients and viscous flux -> SA source -> assembly /V,
 // with the per-block partial sums of R^2 for the L2 norm.
 //
 // Follows EulerEquation::calc_residual (src/model/eulerequation.cpp:202-232)
 // and the kernels it calls: FluidModel::primvars (src/model/fluid.cpp:50-67),
 // BoundaryContainer::apply (src/model/bc.cpp:430-433), ReconstructionSecondOrder
 // (src/model/reconstruction.cpp:55-154), ConvectiveFluxRoe (src/model/flux.cpp:51-146),
-// Mesh::calc_gradient / calc_face (src/utils/mesh.cpp:8-131) and
-// DiffusiveFluxGreenGauss (src/model/flux.cpp:12-48).
+// ConvectiveFluxAUSM (:149-224), Mesh::calc_gradient / calc_face
+// (src/utils/mesh.cpp:8-131) and DiffusiveFluxGreenGauss (src/model/flux.cpp:12-48).
 //
 // Tiling.  A 256-thread block is a 32 x 8 grid of face owners: thread
 // (tx, ty) computes the chi face i0+tx on row j0+ty and the eta face j0+ty on
