@@ -18,6 +18,12 @@ SOURCES = ["sg_api.cpp", "sg_kernels.hip", "sg_linalg.hip", "sg_adjoint.hip", "s
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-fopenmp"]
+# Per-source additions.  The chunked-ILU kernels run one work-group per chunk
+# (a few dozen on the whole chip), so their occupancy does not matter and their
+# level loop is a latency chain: the max-ILP scheduler issues a level's LDS
+# reads together and interleaves its FMA chains, where the default
+# occupancy-driven schedule serialises them.
+SOURCE_FLAGS = {"sg_cilu.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 
 
 def build(verbose: bool = False) -> str:
@@ -29,7 +35,7 @@ def build(verbose: bool = False) -> str:
         if not os.path.exists(path):
             continue
         obj = os.path.join(objdir, src + ".o")
-        cmd = [HIPCC, "-x", "hip", *FLAGS, "-c", path, "-o", obj]
+        cmd = [HIPCC, "-x", "hip", *FLAGS, *SOURCE_FLAGS.get(src, []), "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd))
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

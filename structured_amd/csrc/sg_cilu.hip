@@ -11,13 +11,14 @@
 //
 // Factors are stored in FP32 (the preconditioner only has to approximate
 // A^-1; FGMRES runs in FP64), per chunk in level order, one record per level:
-// rec[((chunk * nlev + L) * NSLOT * B + s * B + e) * W + c] for block entry e
-// of slot s of the cell in column c of the chunk (row j = L - 2c): slot 0 the
-// inverse of the factored diagonal block, lower slots L_rn (unit lower
-// factor), upper slots U_ru.  Cells of a level sit next to each other, so a
-// level's record is read with coalesced loads.  A second copy holds every
-// block transposed, for M^-T: a column of a block is then read like a row,
-// with the same coalesced loads.
+// rec[(((chunk * nlev + L) * NSLOT + s) * NV + r) * W + c) * NV + m] for entry
+// (r, m) of the block in slot s of the cell in column c of the chunk (row
+// j = L - 2c): slot 0 the inverse of the factored diagonal block, lower slots
+// L_rn (unit lower factor), upper slots U_ru.  A block row is NV contiguous
+// floats and the rows r of a level's cells sit next to each other, so the
+// thread for (r, c) reads its row with two wide loads and a wave's loads are
+// coalesced.  A second copy holds every block transposed, for M^-T: a column
+// of a block is then read like a row, with the same loads.
 #include <hip/hip_runtime.h>
 
 #include "sg_linalg.hpp"
@@ -124,8 +125,12 @@ __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int n
     float* RL = R + (size_t)L * RS;
     float* RTL = RT + (size_t)L * RS;
     const int lr = L & (NR - 1);
-    auto at_n = [&](int s) { return (size_t)(s * B + r * NV + q) * W + c; };
-    auto at_t = [&](int s) { return (size_t)(s * B + q * NV + r) * W + c; };
+    // one per-thread offset for every slot and a uniform slot stride (offsets
+    // computed per slot were kept live across the level loop and spilled)
+    constexpr int SS = NV * W * NV;
+    const unsigned on = (unsigned)((r * W + c) * NV + q), ot = (unsigned)((q * W + c) * NV + r);
+    auto at_n = [&](int s) { return (size_t)s * SS + on; };
+    auto at_t = [&](int s) { return (size_t)s * SS + ot; };
 #pragma unroll
     for (int s = 1; s < NSLOT; ++s) {
       const float v = valid ? (float)a[s] : 0.0f;
@@ -187,11 +192,19 @@ __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int n
   }
 }
 
-// One stored row of a block (NV entries W floats apart), widened to double.
+// One stored row of a block (NV contiguous floats), widened to double.  The
+// row is read as one unit so that it takes two wide loads (16 + 4 bytes for
+// NV = 5) rather than NV single-float loads: the level loop issues few
+// enough memory instructions that their address processing does not pace it.
+template <int NV>
+struct FRow {
+  float v[NV];
+};
 template <int NV, int W>
 __device__ __forceinline__ void load_row(const float* p, double* out) {
+  const FRow<NV> row = *reinterpret_cast<const FRow<NV>*>(p);
 #pragma unroll
-  for (int m = 0; m < NV; ++m) out[m] = p[(size_t)m * W];
+  for (int m = 0; m < NV; ++m) out[m] = row.v[m];
 }
 
 // One level's operands of thread (k, c): f[n*NV + m] for the six neighbours n
@@ -210,7 +223,7 @@ __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, in
   const bool lok = L >= 0 && L < nlev;
   const bool valid = lok && c < wk && j >= 0 && j < njc;
   const float* RL = R + (size_t)(lok ? L : 0) * RS;
-  auto row = [&](const float* Rlev, int s, int r, int col) { return Rlev + (size_t)(s * NV + r) * NV * W + col; };
+  auto row = [&](const float* Rlev, int s, int r, int col) { return Rlev + ((size_t)(s * NV + r) * W + col) * NV; };
 #pragma unroll
   for (int n = 0; n < 6; ++n) {
     const int s = ASC ? ct::Low[n] : ct::Up[n];
@@ -232,7 +245,7 @@ __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, in
 template <int NV, int W, bool ASC, bool DIAG>
 __device__ __forceinline__ void cilu_level(int L, int njc, int ldi, size_t plane, int i0, int wk, int k, int c,
                                            const double* f, const double* d, double x, double* z,
-                                           double (*ring)[8][NV], double (*tmp)[NV]) {
+                                           double (*ring)[NV][W], double (*tmp)[W]) {
   constexpr int NR = 8;
   const int j = L - 2 * c;
   const bool valid = c < wk && j >= 0 && j < njc;
@@ -243,45 +256,57 @@ __device__ __forceinline__ void cilu_level(int L, int njc, int ldi, size_t plane
     const int cn = c + ct::Di[s], jn = j + ct::Dj[s];
     // out-of-chunk / out-of-grid neighbours carry f = 0: any in-range ring
     // entry (all finite) will do
-    const double* zr = ring[min(max(cn, 0), W - 1)][jn & (NR - 1)];
+    const int cc = min(max(cn, 0), W - 1), jr = jn & (NR - 1);
     double t = 0.0;
 #pragma unroll
-    for (int m = 0; m < NV; ++m) t += f[n * NV + m] * zr[m];
+    for (int m = 0; m < NV; ++m) t += f[n * NV + m] * ring[jr][m][cc];
     if (n & 1) a1 += t;
     else a0 -= t;
   }
   double acc = a0 - a1;
   if (DIAG) {
-    if (valid) tmp[c][k] = acc;
+    if (valid) tmp[k][c] = acc;
     lds_barrier();
     double v = 0.0;
 #pragma unroll
-    for (int m = 0; m < NV; ++m) v += d[m] * tmp[c][m];
+    for (int m = 0; m < NV; ++m) v += d[m] * tmp[m][c];
     acc = v;
   }
   if (valid) {
-    ring[c][j & (NR - 1)][k] = acc;
+    ring[j & (NR - 1)][k][c] = acc;
     z[k * plane + (size_t)j * ldi + i0 + c] = acc;
   }
   lds_barrier();
 }
 
+// Operand sets in rotation: the loads of a level are issued NBUF - 1 levels
+// ahead of their use.  Two sets measured best; deeper prefetch does not pay,
+// because a level is paced by how fast one CU streams the chunk's factor
+// records (about 42 KB per level at W = 32), not by the latency of one load.
+constexpr int NBUF = 2;
+
 // One triangular pass over the chunk's levels (ascending or descending),
-// applying the diagonal inverse when DIAG.  Two operand sets alternate so that
-// the next level's loads stay in flight while the current level computes.
+// applying the diagonal inverse when DIAG.  NBUF operand sets rotate so that
+// the loads of the next NBUF - 1 levels stay in flight while the current
+// level computes.  The level count is rounded up to a multiple of NBUF; the
+// extra levels hold no valid cell.
 template <int NV, int W, bool TR, bool ASC, bool DIAG>
 __device__ __forceinline__ void cilu_pass(const float* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
-                                          int k, int c, const double* src, double* z, double (*ring)[8][NV],
-                                          double (*tmp)[NV]) {
-  double fa[6 * NV], da[NV], xa, fb[6 * NV], db[NV], xb;
+                                          int k, int c, const double* src, double* z, double (*ring)[NV][W],
+                                          double (*tmp)[W]) {
+  double f[NBUF][6 * NV], d[NBUF][NV], x[NBUF];
   auto lev = [&](int step) { return ASC ? step : nlev - 1 - step; };
-  cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(0), src, fa, da, xa);
-  // an odd level count runs one empty level (no valid cell) at the end
-  for (int step = 0; step < nlev; step += 2) {
-    cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + 1), src, fb, db, xb);
-    cilu_level<NV, W, ASC, DIAG>(lev(step), njc, ldi, plane, i0, wk, k, c, fa, da, xa, z, ring, tmp);
-    cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + 2), src, fa, da, xa);
-    cilu_level<NV, W, ASC, DIAG>(lev(step + 1), njc, ldi, plane, i0, wk, k, c, fb, db, xb, z, ring, tmp);
+#pragma unroll
+  for (int u = 0; u < NBUF - 1; ++u)
+    cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(u), src, f[u], d[u], x[u]);
+  for (int step = 0; step < nlev; step += NBUF) {
+#pragma unroll
+    for (int u = 0; u < NBUF; ++u) {
+      const int nb = (u + NBUF - 1) % NBUF;
+      cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + u + NBUF - 1), src, f[nb], d[nb],
+                                 x[nb]);
+      cilu_level<NV, W, ASC, DIAG>(lev(step + u), njc, ldi, plane, i0, wk, k, c, f[u], d[u], x[u], z, ring, tmp);
+    }
   }
 }
 
@@ -291,8 +316,10 @@ template <int NV, int W, bool TR>
 __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, int ldi, int nlev, const float* rec,
                                                             size_t tcopy, const double* b, double* z) {
   constexpr int RS = cilu_rs<NV, W>();
-  __shared__ double ring[W][8][NV];
-  __shared__ double tmp[W][NV];
+  // column index fastest: the lanes of a wave are consecutive columns, so
+  // their LDS reads and writes fall in distinct banks
+  __shared__ double ring[8][NV][W];
+  __shared__ double tmp[NV][W];
   const int chunk = blockIdx.x, i0 = chunk * W, wk = min(W, nic - i0);
   const size_t plane = (size_t)njc * ldi;
   const int k = threadIdx.x / W, c = threadIdx.x % W;

@@ -26,7 +26,6 @@ int sg_read_grid(const char* path, const char* format, int ni, int nj, double* x
   if (!path || !format || !xv || !yv || ni < 2 || nj < 2) return SG_ERR_ARG;
   std::ifstream in(path);
   if (!in) return sg_fail(nullptr, SG_ERR_ARG, std::string("cannot open grid file ") + path);
-  const size_t n = (size_t)ni * nj;
   // vertex (i, j) is stored at i * nj + j (Mesh::xv[ni][nj]); both formats
   // list the vertices j-outer, i-inner
   if (!std::strcmp(format, "p3d")) {
@@ -45,7 +44,6 @@ int sg_read_grid(const char* path, const char* format, int ni, int nj, double* x
   } else {
     return sg_fail(nullptr, SG_ERR_ARG, "grid format must be \"p3d\" or \"simple\"");
   }
-  (void)n;
   return SG_OK;
 }
 

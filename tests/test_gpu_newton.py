@@ -113,14 +113,15 @@ def test_ilu_matches_numpy_model_sa(precond):
     assert err <= 1e-9
 
 
-@pytest.mark.parametrize("width,sa", [(8, False), (16, False), (8, True)])
+@pytest.mark.parametrize("width,sa", [(8, False), (16, False), (8, True), (32, True)])
 @pytest.mark.parametrize("transpose", [False, True])
 def test_chunked_ilu_matches_numpy_model(width, sa, transpose):
     """Chunked line ILU(0) (precond 6): chunks of `width` columns (a partial
-    last chunk at width 8 on 12 columns), M^-1 and M^-T against the model
-    with float32 factors."""
+    last chunk at width 8 on 12 columns, and at width 32 on 40 columns),
+    M^-1 and M^-T against the model with float32 factors."""
     import precond_model as pm
-    case = cases.sa_bump(13, 9, dy1=1e-3) if sa else cases.laminar_plate(13, 9, dy1=1e-2)
+    ni = 41 if width == 32 else 13
+    case = cases.sa_bump(ni, 9, dy1=1e-3) if sa else cases.laminar_plate(ni, 9, dy1=1e-2)
     q = case.perturbed_state(seed=6, amp=0.01)
     eq = lib.Equation(case)
     dt = eq.calc_dt(q, 50.0)
