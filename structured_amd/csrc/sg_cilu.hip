@@ -21,6 +21,8 @@
 // of a block is then read like a row, with the same loads.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "sg_linalg.hpp"
 
 namespace sg {
@@ -209,13 +211,13 @@ __device__ __forceinline__ void load_row(const float* p, double* out) {
 
 // One level's operands of thread (k, c): f[n*NV + m] for the six neighbours n
 // of the pass, d[m] for the diagonal inverse (row k, or column k when
-// transposed), x the right-hand side.  R is the record copy of the pass: the
+// transposed; only in the pass that applies it), x the right-hand side.  R is the record copy of the pass: the
 // plain one for M^-1, the transposed one for M^-T, where a column of a block
 // is a stored row.  The factorisation stores exact zeros for neighbours
 // outside the chunk or the grid and for invalid cells (the transposed pass
 // reads the cell's own slot, zero for an absent neighbour), so nothing is
 // masked and all the loads of a level are in flight together.
-template <int NV, int W, bool TR, bool ASC>
+template <int NV, int W, bool TR, bool ASC, bool DIAG>
 __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
                                            int k, int c, int L, const double* src, double* f, double* d, double& x) {
   constexpr int RS = cilu_rs<NV, W>();
@@ -237,7 +239,7 @@ __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, in
       load_row<NV, W>(p, f + n * NV);
     }
   }
-  load_row<NV, W>(row(RL, 0, k, c), d);
+  if (DIAG) load_row<NV, W>(row(RL, 0, k, c), d);
   x = src[k * plane + (valid ? (size_t)j * ldi + i0 + c : 0)];
 }
 
@@ -281,58 +283,136 @@ __device__ __forceinline__ void cilu_level(int L, int njc, int ldi, size_t plane
 
 // Operand sets in rotation: the loads of a level are issued NBUF - 1 levels
 // ahead of their use.  Two sets measured best; deeper prefetch does not pay,
-// because a level is paced by how fast one CU streams the chunk's factor
-// records (about 42 KB per level at W = 32), not by the latency of one load.
+// because a level is paced by how fast the chunk's factor records reach one
+// CU (about 22 KB per level and pass at W = 32), not by the latency of one
+// load.
 constexpr int NBUF = 2;
+
+// Helper work-groups: per chunk, NHELP work-groups on the same XCD (work-groups
+// are dealt to the 8 XCDs round-robin by index) load the factor lines of the
+// levels ahead of the solving work-group, so that its loads hit that XCD's L2
+// instead of HBM.  They only load (nothing they do affects the result), at
+// most HLEAD steps ahead of the progress the solving work-group reports every
+// PROG_EVERY steps, HB of their levels per batch.  Their wait for progress is
+// bounded, so they always finish.  One helper measured best (two or three
+// were no faster, six slower); a report costs the solving work-group about a
+// memory round trip, so reports are sparse and the lead is generous.
+constexpr int NHELP = 1, HB = 4, HLEAD = 32, PROG_EVERY = 16;
+constexpr int HSPIN = 4096, HSPIN_TOTAL = 1 << 16;
 
 // One triangular pass over the chunk's levels (ascending or descending),
 // applying the diagonal inverse when DIAG.  NBUF operand sets rotate so that
 // the loads of the next NBUF - 1 levels stay in flight while the current
 // level computes.  The level count is rounded up to a multiple of NBUF; the
-// extra levels hold no valid cell.
+// extra levels hold no valid cell.  Thread 0 reports the steps done to
+// *prog (tag | steps) for the helpers, right after a level's loads, so that
+// the report's store is not waited for before the next level.
 template <int NV, int W, bool TR, bool ASC, bool DIAG>
 __device__ __forceinline__ void cilu_pass(const float* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
                                           int k, int c, const double* src, double* z, double (*ring)[NV][W],
-                                          double (*tmp)[W]) {
+                                          double (*tmp)[W], unsigned* prog, unsigned tag) {
   double f[NBUF][6 * NV], d[NBUF][NV], x[NBUF];
   auto lev = [&](int step) { return ASC ? step : nlev - 1 - step; };
 #pragma unroll
   for (int u = 0; u < NBUF - 1; ++u)
-    cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(u), src, f[u], d[u], x[u]);
+    cilu_fetch<NV, W, TR, ASC, DIAG>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(u), src, f[u], d[u], x[u]);
   for (int step = 0; step < nlev; step += NBUF) {
 #pragma unroll
     for (int u = 0; u < NBUF; ++u) {
       const int nb = (u + NBUF - 1) % NBUF;
-      cilu_fetch<NV, W, TR, ASC>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + u + NBUF - 1), src, f[nb], d[nb],
-                                 x[nb]);
+      cilu_fetch<NV, W, TR, ASC, DIAG>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + u + NBUF - 1), src, f[nb],
+                                       d[nb], x[nb]);
+      if (u == 0 && prog && threadIdx.x == 0 && (step % PROG_EVERY) == 0)
+        __hip_atomic_store(prog, tag + (unsigned)step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       cilu_level<NV, W, ASC, DIAG>(lev(step + u), njc, ldi, plane, i0, wk, k, c, f[u], d[u], x[u], z, ring, tmp);
     }
   }
 }
 
+// Helper h of a chunk: loads one word of every 128-byte line of the slots a
+// pass reads, level by level in the pass's order, steps h, h + NHELP, ...
+// Pass p (0 ascending, 1 descending) of M^-1 reads the lower slots, then the
+// upper slots and the diagonal; M^-T reads them the other way round.
+template <int NV, int W, bool TR>
+__device__ void cilu_helper(const float* R, int nlev, const unsigned* prog, unsigned epoch, int h, unsigned* sink) {
+  constexpr int NT = NV * W, SLOTF = NV * NV * W, RS = cilu_rs<NV, W>();
+  constexpr int LPS = (SLOTF + 31) / 32;          // 128-byte lines per slot
+  constexpr int LPT = (7 * LPS + NT - 1) / NT;    // lines per thread and level
+  float acc = 0.f;
+  int budget = HSPIN_TOTAL;
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool upper = (pass == 0) == TR;  // upper slots and the diagonal
+    const int nslot = upper ? 7 : 6;
+    int off[LPT];
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) {
+      const int t = threadIdx.x + u * NT, si = t / LPS, ln = t % LPS;
+      const int s = si >= nslot ? -1 : upper ? (si == 6 ? 0 : ct::Up[si]) : ct::Low[si];
+      off[u] = (s < 0 || ln * 32 >= SLOTF) ? -1 : s * SLOTF + ln * 32;
+    }
+    const unsigned base = (unsigned)(pass * nlev);
+    for (int s0 = h; s0 < nlev; s0 += NHELP * HB) {
+      const unsigned last = base + (unsigned)min(s0 + NHELP * (HB - 1), nlev - 1);
+      for (int spin = 0; spin < HSPIN && budget > 0; ++spin, --budget) {
+        const unsigned v = __hip_atomic_load(prog, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned done = (v >> 16) == epoch ? (v & 0xffffu) : 0u;
+        if (done + HLEAD >= last) break;
+        __builtin_amdgcn_s_sleep(4);
+      }
+      float v[HB][LPT];
+#pragma unroll
+      for (int b = 0; b < HB; ++b) {
+        const int step = s0 + NHELP * b, L = pass == 0 ? step : nlev - 1 - step;
+#pragma unroll
+        for (int u = 0; u < LPT; ++u) v[b][u] = (step < nlev && off[u] >= 0) ? R[(size_t)L * RS + off[u]] : 0.f;
+      }
+#pragma unroll
+      for (int b = 0; b < HB; ++b)
+#pragma unroll
+        for (int u = 0; u < LPT; ++u) acc += v[b][u];
+    }
+  }
+  // keeps the loads; the factors are finite, so this never stores
+  if (acc != acc && threadIdx.x == 0) *sink = 1u;
+}
+
 // z = (LU)^-1 b, or (LU)^-T b when TR.  Thread (k, c): component k of the
 // cell in column c.  Operands are loaded one level ahead of their use.
+// Work-groups [0, nchunk) solve their chunk; with helpers (prog non-null),
+// work-group npad (1 + m) + chunk is helper m of the chunk, npad being the
+// chunk count rounded up to a multiple of 8, so it runs on the chunk's XCD.
 template <int NV, int W, bool TR>
 __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, int ldi, int nlev, const float* rec,
-                                                            size_t tcopy, const double* b, double* z) {
+                                                            size_t tcopy, const double* b, double* z, unsigned* prog,
+                                                            unsigned epoch) {
   constexpr int RS = cilu_rs<NV, W>();
   // column index fastest: the lanes of a wave are consecutive columns, so
   // their LDS reads and writes fall in distinct banks
   __shared__ double ring[8][NV][W];
   __shared__ double tmp[NV][W];
-  const int chunk = blockIdx.x, i0 = chunk * W, wk = min(W, nic - i0);
+  const int nchunk = (nic + W - 1) / W, npad = (nchunk + 7) & ~7;
+  const int chunk = (int)blockIdx.x % npad, role = (int)blockIdx.x / npad;
+  if (chunk >= nchunk) return;
+  const float* R = rec + (TR ? tcopy : 0) + (size_t)chunk * nlev * RS;
+  if (role > 0) {
+    cilu_helper<NV, W, TR>(R, nlev, prog + chunk, epoch, role - 1, prog + npad);
+    return;
+  }
+  const int i0 = chunk * W, wk = min(W, nic - i0);
   const size_t plane = (size_t)njc * ldi;
   const int k = threadIdx.x / W, c = threadIdx.x % W;
-  const float* R = rec + (TR ? tcopy : 0) + (size_t)chunk * nlev * RS;
+  unsigned* pc = prog ? prog + chunk : nullptr;
+  const unsigned tag = epoch << 16;
   // ring cells never written hold zeros (their f is zero too)
   for (int e = threadIdx.x; e < W * 8 * NV; e += NV * W) (&ring[0][0][0])[e] = 0.0;
   __syncthreads();
   // non-transposed: unit-lower L ascending, then D + U descending;
   // transposed: (D + U)^T ascending, then unit-lower L^T descending
-  cilu_pass<NV, W, TR, true, TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, b, z, ring, tmp);
+  cilu_pass<NV, W, TR, true, TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, b, z, ring, tmp, pc, tag);
   // the second pass reads what the first stored: complete every store first
   __syncthreads();
-  cilu_pass<NV, W, TR, false, !TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, z, z, ring, tmp);
+  cilu_pass<NV, W, TR, false, !TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, z, z, ring, tmp, pc,
+                                   tag + (unsigned)nlev);
 }
 
 // floats of one record copy (plain or transposed)
@@ -341,8 +421,14 @@ static size_t cilu_copy_floats(int nic, int njc, int nv, int w) {
   return (size_t)nchunk * nlev * NSLOT * nv * nv * w;
 }
 
-// in doubles of storage: both copies of FP32 factors
-size_t cilu_size(int nic, int njc, int nv, int w) { return cilu_copy_floats(nic, njc, nv, w); }
+// progress words of the helpers: one per chunk (padded to a multiple of 8)
+// and one sink, after both copies
+static size_t cilu_prog_words(int nic, int w) { return (size_t)(((nic + w - 1) / w + 7) & ~7) + 1; }
+
+// in doubles of storage: both copies of FP32 factors, then the progress words
+size_t cilu_size(int nic, int njc, int nv, int w) {
+  return cilu_copy_floats(nic, njc, nv, w) + (cilu_prog_words(nic, w) + 1) / 2;
+}
 
 template <int NV, int W>
 static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const double* blocks, float* rec,
@@ -352,17 +438,26 @@ static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const do
                      blocks, rec, cilu_copy_floats(nic, njc, NV, W));
 }
 
+// Launch tag of an apply: helpers only trust progress words that carry it.
+static std::atomic<unsigned> g_apply_epoch{0};
+
 template <int NV, int W>
 static void cilu_apply_t(int nic, int njc, int ldi, const float* rec, const double* b, double* z, bool tr,
                          hipStream_t s) {
-  const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2;
+  const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2, npad = (nchunk + 7) & ~7;
   const size_t tcopy = cilu_copy_floats(nic, njc, NV, W);
+  unsigned* prog = reinterpret_cast<unsigned*>(const_cast<float*>(rec) + 2 * tcopy);
+  // the progress word holds tag << 16 | steps; without room for 2 nlev steps, no helpers
+  const bool helpers = 2 * nlev < (1 << 16);
+  unsigned epoch = (g_apply_epoch.fetch_add(1) % 0xffffu) + 1;  // never 0, the value of fresh memory
+  const dim3 grid(helpers ? npad * (1 + NHELP) : nchunk);
+  if (!helpers) prog = nullptr, epoch = 0;
   if (tr)
-    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, true>), dim3(nchunk), dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec,
-                       tcopy, b, z);
+    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, true>), grid, dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec, tcopy, b,
+                       z, prog, epoch);
   else
-    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, false>), dim3(nchunk), dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec,
-                       tcopy, b, z);
+    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, false>), grid, dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec, tcopy,
+                       b, z, prog, epoch);
 }
 
 bool cilu_width_ok(int w) { return w == 8 || w == 16 || w == 32; }
