@@ -244,10 +244,10 @@ __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, in
 }
 
 // One level of a triangular pass with its operands already in registers.
+// Returns the thread's solved value (stored to z by the caller).
 template <int NV, int W, bool ASC, bool DIAG>
-__device__ __forceinline__ void cilu_level(int L, int njc, int ldi, size_t plane, int i0, int wk, int k, int c,
-                                           const double* f, const double* d, double x, double* z,
-                                           double (*ring)[NV][W], double (*tmp)[W]) {
+__device__ __forceinline__ double cilu_level(int L, int njc, int wk, int k, int c, const double* f, const double* d,
+                                             double x, double (*ring)[NV][W], double (*tmp)[W]) {
   constexpr int NR = 8;
   const int j = L - 2 * c;
   const bool valid = c < wk && j >= 0 && j < njc;
@@ -274,11 +274,9 @@ __device__ __forceinline__ void cilu_level(int L, int njc, int ldi, size_t plane
     for (int m = 0; m < NV; ++m) v += d[m] * tmp[m][c];
     acc = v;
   }
-  if (valid) {
-    ring[j & (NR - 1)][k][c] = acc;
-    z[k * plane + (size_t)j * ldi + i0 + c] = acc;
-  }
+  if (valid) ring[j & (NR - 1)][k][c] = acc;
   lds_barrier();
+  return acc;
 }
 
 // Operand sets in rotation: the loads of a level are issued NBUF - 1 levels
@@ -300,10 +298,18 @@ constexpr int NBUF = 2;
 constexpr int NHELP = 1, HB = 4, HLEAD = 32, PROG_EVERY = 16;
 constexpr int HSPIN = 4096, HSPIN_TOTAL = 1 << 16;
 
+// Levels whose solved values are stored to z together.  A store joins the
+// in-order count of outstanding memory operations that every later wait for
+// a load drains, so one burst per ZB levels stalls at most once where a
+// store per level could stall every level.  With the helper in place the
+// measured effect is small (M^-T 603 -> 594 us, M^-1 unchanged at 705x321).
+constexpr int ZB = 8;
+static_assert(ZB % NBUF == 0 && PROG_EVERY % ZB == 0, "batch sizes");
+
 // One triangular pass over the chunk's levels (ascending or descending),
 // applying the diagonal inverse when DIAG.  NBUF operand sets rotate so that
 // the loads of the next NBUF - 1 levels stay in flight while the current
-// level computes.  The level count is rounded up to a multiple of NBUF; the
+// level computes.  The level count is rounded up to a multiple of ZB; the
 // extra levels hold no valid cell.  Thread 0 reports the steps done to
 // *prog (tag | steps) for the helpers, right after a level's loads, so that
 // the report's store is not waited for before the next level.
@@ -316,15 +322,21 @@ __device__ __forceinline__ void cilu_pass(const float* R, int nlev, int njc, int
 #pragma unroll
   for (int u = 0; u < NBUF - 1; ++u)
     cilu_fetch<NV, W, TR, ASC, DIAG>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(u), src, f[u], d[u], x[u]);
-  for (int step = 0; step < nlev; step += NBUF) {
+  for (int step = 0; step < nlev; step += ZB) {
+    double zv[ZB];
 #pragma unroll
-    for (int u = 0; u < NBUF; ++u) {
-      const int nb = (u + NBUF - 1) % NBUF;
+    for (int u = 0; u < ZB; ++u) {
+      const int cur = u % NBUF, nb = (u + NBUF - 1) % NBUF;
       cilu_fetch<NV, W, TR, ASC, DIAG>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + u + NBUF - 1), src, f[nb],
                                        d[nb], x[nb]);
       if (u == 0 && prog && threadIdx.x == 0 && (step % PROG_EVERY) == 0)
         __hip_atomic_store(prog, tag + (unsigned)step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      cilu_level<NV, W, ASC, DIAG>(lev(step + u), njc, ldi, plane, i0, wk, k, c, f[u], d[u], x[u], z, ring, tmp);
+      zv[u] = cilu_level<NV, W, ASC, DIAG>(lev(step + u), njc, wk, k, c, f[cur], d[cur], x[cur], ring, tmp);
+    }
+#pragma unroll
+    for (int u = 0; u < ZB; ++u) {
+      const int j = lev(step + u) - 2 * c;
+      if (c < wk && j >= 0 && j < njc) z[k * plane + (size_t)j * ldi + i0 + c] = zv[u];
     }
   }
 }
