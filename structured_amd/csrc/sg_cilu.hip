@@ -384,7 +384,8 @@ __device__ void cilu_helper(const float* R, int nlev, const unsigned* prog, unsi
         for (int u = 0; u < LPT; ++u) acc += v[b][u];
     }
   }
-  // keeps the loads; the factors are finite, so this never stores
+  // keeps the loads: the sum is used, but it reaches memory only when a factor
+  // is not finite, and then only a word that nothing reads
   if (acc != acc && threadIdx.x == 0) *sink = 1u;
 }
 
