@@ -21,6 +21,7 @@
 // of a block is then read like a row, with the same loads.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 
 #include "sg_linalg.hpp"
@@ -56,41 +57,76 @@ constexpr int slot(int di, int dj) {
 // every stencil block of its cell's row in registers.  A row's lower
 // neighbours lie on the four previous levels (L-1 .. L-4), whose factored
 // diagonal inverses and upper blocks are kept in an LDS ring, so the level
-// loop reads the 13 Jacobian blocks of its own cells from memory and nothing
-// else; the elimination exchanges one 5x5 block per cell through LDS per
-// lower neighbour.
+// loop reads the 13 blocks of its own cells (gathered in level order by
+// cilu_gather_kernel) from memory and nothing else; the elimination exchanges
+// one 5x5 block per cell through LDS per lower neighbour.
 // Record size of one level, in floats.
 template <int NV, int W>
 constexpr int cilu_rs() { return NSLOT * NV * NV * W; }
 
+// Gathers the factorisation's operands into level order: for chunk, level L,
+// slot s, block entry e and column c, entry e of the block of I/dt - J in row
+// (i0 + c, L - 2c) and neighbour slot s, or zero where the cell or the
+// neighbour lies outside the chunk or the grid.  Consecutive columns are
+// contiguous, so the factorisation reads a level's operands in runs of W
+// doubles.  Read in place, a level's W cells lie on a diagonal of rows 2 ldi
+// apart: every 8-byte read took its own line, used again only ~W levels later
+// after L2 had evicted it.  Here all levels are gathered at once and the
+// levels that share a line are read together.
 template <int NV, int W>
-__global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int njc, int ldi, int nlev, const double* idt,
-                                                                  const double* blocks, float* rec, size_t tcopy) {
+__global__ __launch_bounds__(256) void cilu_gather_kernel(int nic, int njc, int ldi, int nlev, const double* idt,
+                                                          const double* blocks, double* ops, size_t total) {
+  constexpr int B = NV * NV;
+  const size_t plane = (size_t)njc * ldi;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    size_t u = t;
+    const int c = (int)(u % W);
+    u /= W;
+    const int e = (int)(u % B);
+    u /= B;
+    const int s = (int)(u % NSLOT);
+    u /= NSLOT;
+    const int L = (int)(u % (size_t)nlev), chunk = (int)(u / (size_t)nlev);
+    const int i0 = chunk * W, wk = min(W, nic - i0), j = L - 2 * c;
+    const int cn = c + ct::Di[s], jn = j + ct::Dj[s];
+    double v = 0.0;
+    if (c < wk && j >= 0 && j < njc && cn >= 0 && cn < wk && jn >= 0 && jn < njc) {
+      const size_t cell = (size_t)j * ldi + i0 + c;
+      v = -blocks[(size_t)(s * B + e) * plane + cell];
+      if (s == 0 && e / NV == e % NV) v += idt[cell];
+    }
+    ops[t] = v;
+  }
+}
+
+// floats of one record copy (plain or transposed)
+static size_t cilu_copy_floats(int nic, int njc, int nv, int w) {
+  const int nchunk = (nic + w - 1) / w, nlev = njc + 2 * w - 2;
+  return (size_t)nchunk * nlev * NSLOT * nv * nv * w;
+}
+
+template <int NV, int W>
+__global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int njc, int nlev, const double* ops,
+                                                                  float* rec, size_t tcopy) {
   constexpr int B = NV * NV, RS = cilu_rs<NV, W>(), NR = 4;
   __shared__ float ring[NR][7][W][B];  // per level: diagonal inverse, then the six upper blocks (ct::Up order)
   __shared__ double Ab[W][B];
   __shared__ double Lb[W][B];
   __shared__ double cb[2][W][NV];
   const int chunk = blockIdx.x, i0 = chunk * W, wk = min(W, nic - i0);
-  const size_t plane = (size_t)njc * ldi;
   const int t = threadIdx.x, c = t / B, e = t % B, r = e / NV, q = e % NV;
   float* R = rec + (size_t)chunk * nlev * RS;
   float* RT = R + tcopy;
   auto inside = [&](int cc, int jj) { return cc >= 0 && cc < wk && jj >= 0 && jj < njc; };
   for (int L = 0; L < nlev; ++L) {
-    const int j = L - 2 * c, i = i0 + c;
+    const int j = L - 2 * c;
     const bool valid = inside(c, j);
-    const size_t cell = valid ? (size_t)j * ldi + i : 0;
+    // this row's 13 blocks of I/dt - J, gathered in level order (zero where
+    // the cell or the neighbour is outside)
+    const double* opl = ops + ((size_t)(chunk * nlev + L) * NSLOT * B + e) * W + c;
     double a[NSLOT];
 #pragma unroll
-    for (int s = 0; s < NSLOT; ++s) {
-      double v = 0.0;
-      if (valid && inside(c + ct::Di[s], j + ct::Dj[s])) {
-        v = -blocks[(size_t)(s * B + e) * plane + cell];
-        if (s == 0 && r == q) v += idt[cell];
-      }
-      a[s] = v;
-    }
+    for (int s = 0; s < NSLOT; ++s) a[s] = opl[(size_t)s * B * W];
 #pragma unroll
     for (int qi = 0; qi < 6; ++qi) {
       const int s = ct::Low[qi], di = ct::Di[s], dj = ct::Dj[s];
@@ -428,27 +464,33 @@ __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, in
                                    tag + (unsigned)nlev);
 }
 
-// floats of one record copy (plain or transposed)
-static size_t cilu_copy_floats(int nic, int njc, int nv, int w) {
-  const int nchunk = (nic + w - 1) / w, nlev = njc + 2 * w - 2;
-  return (size_t)nchunk * nlev * NSLOT * nv * nv * w;
-}
-
 // progress words of the helpers: one per chunk (padded to a multiple of 8)
 // and one sink, after both copies
 static size_t cilu_prog_words(int nic, int w) { return (size_t)(((nic + w - 1) / w + 7) & ~7) + 1; }
 
-// in doubles of storage: both copies of FP32 factors, then the progress words
-size_t cilu_size(int nic, int njc, int nv, int w) {
+// doubles before the gathered factorisation operands: both copies of FP32
+// factors, then the progress words
+static size_t cilu_ops_offset(int nic, int njc, int nv, int w) {
   return cilu_copy_floats(nic, njc, nv, w) + (cilu_prog_words(nic, w) + 1) / 2;
 }
 
+// in doubles of storage: the factors, the progress words and the gathered
+// operands (as many doubles as one copy has floats)
+size_t cilu_size(int nic, int njc, int nv, int w) {
+  return cilu_ops_offset(nic, njc, nv, w) + cilu_copy_floats(nic, njc, nv, w);
+}
+
 template <int NV, int W>
-static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const double* blocks, float* rec,
+static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const double* blocks, double* store,
                           hipStream_t s) {
   const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2;
-  hipLaunchKernelGGL((cilu_factor_kernel<NV, W>), dim3(nchunk), dim3(W * NV * NV), 0, s, nic, njc, ldi, nlev, idt,
-                     blocks, rec, cilu_copy_floats(nic, njc, NV, W));
+  const size_t total = cilu_copy_floats(nic, njc, NV, W);
+  double* ops = store + cilu_ops_offset(nic, njc, NV, W);
+  const size_t nblk = std::min<size_t>((total + 255) / 256, (size_t)1 << 20);
+  hipLaunchKernelGGL((cilu_gather_kernel<NV, W>), dim3((unsigned)nblk), dim3(256), 0, s, nic, njc, ldi, nlev, idt,
+                     blocks, ops, total);
+  hipLaunchKernelGGL((cilu_factor_kernel<NV, W>), dim3(nchunk), dim3(W * NV * NV), 0, s, nic, njc, nlev, ops,
+                     reinterpret_cast<float*>(store), total);
 }
 
 // Launch tag of an apply: helpers only trust progress words that carry it.
@@ -477,15 +519,14 @@ bool cilu_width_ok(int w) { return w == 8 || w == 16 || w == 32; }
 
 hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, const double* idt, const double* blocks, double* store,
                        hipStream_t s) {
-  float* rec = reinterpret_cast<float*>(store);
   if (nv == 5) {
-    if (w == 8) cilu_factor_t<5, 8>(nic, njc, ldi, idt, blocks, rec, s);
-    else if (w == 16) cilu_factor_t<5, 16>(nic, njc, ldi, idt, blocks, rec, s);
-    else cilu_factor_t<5, 32>(nic, njc, ldi, idt, blocks, rec, s);
+    if (w == 8) cilu_factor_t<5, 8>(nic, njc, ldi, idt, blocks, store, s);
+    else if (w == 16) cilu_factor_t<5, 16>(nic, njc, ldi, idt, blocks, store, s);
+    else cilu_factor_t<5, 32>(nic, njc, ldi, idt, blocks, store, s);
   } else {
-    if (w == 8) cilu_factor_t<4, 8>(nic, njc, ldi, idt, blocks, rec, s);
-    else if (w == 16) cilu_factor_t<4, 16>(nic, njc, ldi, idt, blocks, rec, s);
-    else cilu_factor_t<4, 32>(nic, njc, ldi, idt, blocks, rec, s);
+    if (w == 8) cilu_factor_t<4, 8>(nic, njc, ldi, idt, blocks, store, s);
+    else if (w == 16) cilu_factor_t<4, 16>(nic, njc, ldi, idt, blocks, store, s);
+    else cilu_factor_t<4, 32>(nic, njc, ldi, idt, blocks, store, s);
   }
   return hipGetLastError();
 }

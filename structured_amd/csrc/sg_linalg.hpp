@@ -41,8 +41,9 @@ hipError_t lilu_factor(int nic, int njc, int ldi, int nv, const double* idt, con
 hipError_t lilu_apply(int nic, int njc, int ldi, int nv, const double* ilu, const double* b, double* z, bool transpose,
                       hipStream_t s);
 // chunked line block ILU(0) (sg_cilu.hip): chunks of w in {8, 16, 32} columns;
-// cilu_size is the factor storage in doubles (two FP32 copies, plain and
-// transposed), which cilu_factor writes and cilu_apply reads
+// cilu_size is the storage in doubles that cilu_factor writes and cilu_apply
+// reads: two FP32 copies of the factors (plain and transposed), the apply's
+// progress words, and the factorisation's operands gathered in level order
 bool cilu_width_ok(int w);
 size_t cilu_size(int nic, int njc, int nv, int w);
 hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, const double* idt, const double* blocks, double* rec,
