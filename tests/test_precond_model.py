@@ -1,0 +1,101 @@
+"""Known answers for the numpy preconditioner model (tests/precond_model.py)
+that the device's chunked line ILU is checked against: where the kept
+pattern holds all of the exact LU's fill, block ILU(0) is the exact inverse;
+couplings between chunks are dropped, which makes one-column chunks a block
+Jacobi over columns with exact column solves."""
+import numpy as np
+
+import precond_model as pm
+
+
+def _random_blocks(nic, njc, nv, seed, drop_columns=False):
+    """Random stencil blocks J (13 slots) with a dominant shift idt, so that
+    A = I idt - J is well conditioned.  drop_columns: no coupling between
+    different columns (slots with di != 0 are zero)."""
+    rng = np.random.default_rng(seed)
+    B = 0.1 * rng.standard_normal((nic, njc, len(pm.SLOTS), nv, nv))
+    for s, (di, dj) in enumerate(pm.SLOTS):
+        if drop_columns and di != 0:
+            B[:, :, s] = 0.0
+        for i in range(nic):
+            for j in range(njc):
+                if not (0 <= i + di < nic and 0 <= j + dj < njc):
+                    B[i, j, s] = 0.0
+    idt = 5.0 + rng.random((nic, njc))
+    return B, idt
+
+
+def _dense(B, idt):
+    """A = I idt - J as a dense matrix, unknown (i, j, k) at (i njc + j) nv + k."""
+    nic, njc, ns, nv, _ = B.shape
+    A = np.zeros((nic * njc * nv, nic * njc * nv))
+    for i in range(nic):
+        for j in range(njc):
+            r = (i * njc + j) * nv
+            A[r:r + nv, r:r + nv] += np.eye(nv) * idt[i, j]
+            for s, (di, dj) in enumerate(pm.SLOTS):
+                ii, jj = i + di, j + dj
+                if 0 <= ii < nic and 0 <= jj < njc:
+                    c = (ii * njc + jj) * nv
+                    A[r:r + nv, c:c + nv] -= B[i, j, s]
+    return A
+
+
+def test_chunked_ilu_is_exact_on_one_column():
+    """One column: the matrix is block banded with half-bandwidth 2, its LU
+    has no fill outside the band, so ILU(0) on the 13-point pattern is the
+    exact LU; M^-1 and M^-T equal A^-1 and A^-T."""
+    nic, njc, nv = 1, 14, 5
+    B, idt = _random_blocks(nic, njc, nv, seed=1)
+    A = _dense(B, idt)
+    v = np.random.default_rng(2).standard_normal((nic, njc, nv))
+    for tr in (False, True):
+        z = pm.ilu_chunked(B, idt, v, 8, transpose=tr, f32=False)
+        ref = np.linalg.solve(A.T if tr else A, v.reshape(-1)).reshape(v.shape)
+        assert np.abs(z - ref).max() <= 1e-12 * np.abs(ref).max(), tr
+
+
+def test_one_column_chunks_are_block_jacobi_over_columns():
+    """Chunks of one column drop every coupling between columns: the model
+    equals the exact solve of the column-block-diagonal part of A, and
+    differs from A^-1 when columns are coupled."""
+    nic, njc, nv = 4, 9, 5
+    B, idt = _random_blocks(nic, njc, nv, seed=3)
+    Bd = B.copy()
+    for s, (di, dj) in enumerate(pm.SLOTS):
+        if di != 0:
+            Bd[:, :, s] = 0.0
+    A, Ad = _dense(B, idt), _dense(Bd, idt)
+    v = np.random.default_rng(4).standard_normal((nic, njc, nv))
+    for tr in (False, True):
+        z = pm.ilu_chunked(B, idt, v, 1, transpose=tr, f32=False)
+        ref = np.linalg.solve(Ad.T if tr else Ad, v.reshape(-1)).reshape(v.shape)
+        assert np.abs(z - ref).max() <= 1e-12 * np.abs(ref).max(), tr
+        full = np.linalg.solve(A.T if tr else A, v.reshape(-1)).reshape(v.shape)
+        assert np.abs(z - full).max() > 1e-4 * np.abs(full).max(), tr
+
+
+def test_chunks_spanning_the_grid_without_cross_coupling_are_exact():
+    """Without couplings between columns the chunk width does not matter:
+    every chunk width gives the exact inverse of A (each column is exact,
+    and ILU(0) of a block-diagonal matrix of banded columns is exact)."""
+    nic, njc, nv = 5, 8, 4
+    B, idt = _random_blocks(nic, njc, nv, seed=5, drop_columns=True)
+    A = _dense(B, idt)
+    v = np.random.default_rng(6).standard_normal((nic, njc, nv))
+    ref = np.linalg.solve(A, v.reshape(-1)).reshape(v.shape)
+    for w in (1, 2, 8):
+        z = pm.ilu_chunked(B, idt, v, w, f32=False)
+        assert np.abs(z - ref).max() <= 1e-12 * np.abs(ref).max(), w
+
+
+def test_float32_factors_round_at_float32_level():
+    """Storing the factors in float32 changes the result by about float32
+    rounding (not more, and not zero)."""
+    nic, njc, nv = 3, 10, 5
+    B, idt = _random_blocks(nic, njc, nv, seed=7)
+    v = np.random.default_rng(8).standard_normal((nic, njc, nv))
+    z64 = pm.ilu_chunked(B, idt, v, 8, f32=False)
+    z32 = pm.ilu_chunked(B, idt, v, 8, f32=True)
+    rel = np.abs(z32 - z64).max() / np.abs(z64).max()
+    assert 1e-10 < rel < 1e-5
