@@ -1,8 +1,10 @@
 """Known answers for the numpy preconditioner model (tests/precond_model.py)
-that the device's chunked line ILU is checked against: where the kept
-pattern holds all of the exact LU's fill, block ILU(0) is the exact inverse;
-couplings between chunks are dropped, which makes one-column chunks a block
-Jacobi over columns with exact column solves."""
+that the device's preconditioners (block Jacobi, j-line, multicolour line
+Gauss-Seidel, five-colour, natural-order and chunked line ILU) are checked
+against.  Each one is the exact inverse on the matrices whose pattern it
+captures completely: where the kept pattern holds all of the exact LU's
+fill, block ILU(0) is the exact inverse; couplings between chunks are
+dropped, which makes one-column chunks a block Jacobi over columns."""
 import numpy as np
 
 import precond_model as pm
@@ -99,3 +101,78 @@ def test_float32_factors_round_at_float32_level():
     z32 = pm.ilu_chunked(B, idt, v, 8, f32=True)
     rel = np.abs(z32 - z64).max() / np.abs(z64).max()
     assert 1e-10 < rel < 1e-5
+
+
+def _keep_slots(B, slots):
+    Bk = np.zeros_like(B)
+    Bk[:, :, slots] = B[:, :, slots]
+    return Bk
+
+
+def _exact(B, idt, v):
+    return np.linalg.solve(_dense(B, idt), v.reshape(-1)).reshape(v.shape)
+
+
+def test_block_jacobi_and_line_models_are_exact_on_their_own_pattern():
+    """Block Jacobi (1) is A^-1 when J holds only the diagonal blocks; the
+    j-line solve (2) is A^-1 when J holds only the diagonal and the j +- 1
+    blocks; with couplings outside those patterns they are not."""
+    nic, njc, nv = 4, 7, 5
+    B, idt = _random_blocks(nic, njc, nv, seed=9)
+    v = np.random.default_rng(10).standard_normal((nic, njc, nv))
+    for precond, slots in ((1, [0]), (2, [0, 1, 2])):
+        Bk = _keep_slots(B, slots)
+        ref = _exact(Bk, idt, v)
+        z = pm.apply(Bk, idt, v, precond)
+        assert np.abs(z - ref).max() <= 1e-12 * np.abs(ref).max(), precond
+        full = _exact(B, idt, v)
+        assert np.abs(pm.apply(B, idt, v, precond) - full).max() > 1e-4 * np.abs(full).max(), precond
+
+
+def test_multicolour_line_gauss_seidel_converges_to_the_inverse():
+    """The symmetric three-colour j-line Gauss-Seidel (3): one sweep is exact
+    without couplings between lines, and repeated sweeps on a diagonally
+    dominant A converge to A^-1 v."""
+    nic, njc, nv = 6, 5, 4
+    B, idt = _random_blocks(nic, njc, nv, seed=11)
+    v = np.random.default_rng(12).standard_normal((nic, njc, nv))
+    Bl = _keep_slots(B, [0, 1, 2])
+    ref = _exact(Bl, idt, v)
+    assert np.abs(pm.apply(Bl, idt, v, 3, sweeps=1) - ref).max() <= 1e-12 * np.abs(ref).max()
+    ref = _exact(B, idt, v)
+    one = np.abs(pm.apply(B, idt, v, 3, sweeps=1) - ref).max()
+    many = np.abs(pm.apply(B, idt, v, 3, sweeps=40) - ref).max()
+    assert many <= 1e-10 * np.abs(ref).max() and one > 1e3 * many
+
+
+def test_five_colour_ilu_is_exact_for_colour_lower_triangular_matrices():
+    """Five-colour ILU(0) (4): when every coupling points to a lower colour,
+    A is block lower triangular in colour order, its LU has no fill and the
+    model is A^-1; with couplings both ways it is not."""
+    nic, njc, nv = 6, 6, 4
+    B, idt = _random_blocks(nic, njc, nv, seed=13)
+    Bl = B.copy()
+    for i in range(nic):
+        for j in range(njc):
+            for s, (di, dj) in enumerate(pm.SLOTS):
+                if s and not (0 <= i + di < nic and 0 <= j + dj < njc and pm.colour5(i + di, j + dj) < pm.colour5(i, j)):
+                    Bl[i, j, s] = 0.0
+    v = np.random.default_rng(14).standard_normal((nic, njc, nv))
+    ref = _exact(Bl, idt, v)
+    assert np.abs(pm.ilu_mc5(Bl, idt, v) - ref).max() <= 1e-12 * np.abs(ref).max()
+    full = _exact(B, idt, v)
+    assert np.abs(pm.ilu_mc5(B, idt, v) - full).max() > 1e-4 * np.abs(full).max()
+
+
+def test_natural_order_ilu_is_exact_on_one_row():
+    """Natural-order ILU(0) (5) orders cells j-major; on a grid one cell high
+    the matrix is block banded with half-bandwidth 2 and the model is A^-1
+    (M^-1 and M^-T)."""
+    nic, njc, nv = 12, 1, 5
+    B, idt = _random_blocks(nic, njc, nv, seed=15)
+    A = _dense(B, idt)
+    v = np.random.default_rng(16).standard_normal((nic, njc, nv))
+    for tr in (False, True):
+        ref = np.linalg.solve(A.T if tr else A, v.reshape(-1)).reshape(v.shape)
+        z = pm.ilu_natural(B, idt, v, transpose=tr)
+        assert np.abs(z - ref).max() <= 1e-12 * np.abs(ref).max(), tr
