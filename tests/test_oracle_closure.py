@@ -173,3 +173,24 @@ def test_bf16_rounding_model():
     assert r[4] == -2.5
     assert r[5] == 3.140625
     assert abs(r[6] - 1e-3) <= 2 ** -9 * 1e-3
+
+
+def test_adam_model_matches_torch_adam():
+    """The model's Adam step (Kingma & Ba, bias-corrected moments) against
+    torch.optim.Adam with the same hyper-parameters, over several steps in
+    FP64 on the CPU."""
+    import torch
+    rng = np.random.default_rng(5)
+    p0 = rng.standard_normal(50)
+    grads = [rng.standard_normal(50) for _ in range(4)]
+    p, m, v = p0.copy(), np.zeros(50), np.zeros(50)
+    for step, g in enumerate(grads, start=1):
+        p, m, v = mm.adam(p, m, v, g, step, lr=1e-3)
+    tp = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([tp], lr=1e-3, betas=(0.9, 0.999), eps=1e-8)
+    for g in grads:
+        opt.zero_grad()
+        tp.grad = torch.tensor(g, dtype=torch.float64)
+        opt.step()
+    np.testing.assert_allclose(p, tp.detach().numpy(), rtol=0, atol=1e-15)
+    assert np.abs(p - p0).max() > 1e-4
