@@ -497,11 +497,13 @@ static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const do
 static std::atomic<unsigned> g_apply_epoch{0};
 
 template <int NV, int W>
-static void cilu_apply_t(int nic, int njc, int ldi, const float* rec, const double* b, double* z, bool tr,
+static void cilu_apply_t(int nic, int njc, int ldi, double* store, const double* b, double* z, bool tr,
                          hipStream_t s) {
   const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2, npad = (nchunk + 7) & ~7;
   const size_t tcopy = cilu_copy_floats(nic, njc, NV, W);
-  unsigned* prog = reinterpret_cast<unsigned*>(const_cast<float*>(rec) + 2 * tcopy);
+  const float* rec = reinterpret_cast<const float*>(store);
+  // the progress words follow the two factor copies; the apply writes them
+  unsigned* prog = reinterpret_cast<unsigned*>(reinterpret_cast<float*>(store) + 2 * tcopy);
   // the progress word holds tag << 16 | steps; without room for 2 nlev steps, no helpers
   const bool helpers = 2 * nlev < (1 << 16);
   unsigned epoch = (g_apply_epoch.fetch_add(1) % 0xffffu) + 1;  // never 0, the value of fresh memory
@@ -531,17 +533,16 @@ hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, const double* i
   return hipGetLastError();
 }
 
-hipError_t cilu_apply(int nic, int njc, int ldi, int nv, int w, const double* store, const double* b, double* z,
+hipError_t cilu_apply(int nic, int njc, int ldi, int nv, int w, double* store, const double* b, double* z,
                       bool transpose, hipStream_t s) {
-  const float* rec = reinterpret_cast<const float*>(store);
   if (nv == 5) {
-    if (w == 8) cilu_apply_t<5, 8>(nic, njc, ldi, rec, b, z, transpose, s);
-    else if (w == 16) cilu_apply_t<5, 16>(nic, njc, ldi, rec, b, z, transpose, s);
-    else cilu_apply_t<5, 32>(nic, njc, ldi, rec, b, z, transpose, s);
+    if (w == 8) cilu_apply_t<5, 8>(nic, njc, ldi, store, b, z, transpose, s);
+    else if (w == 16) cilu_apply_t<5, 16>(nic, njc, ldi, store, b, z, transpose, s);
+    else cilu_apply_t<5, 32>(nic, njc, ldi, store, b, z, transpose, s);
   } else {
-    if (w == 8) cilu_apply_t<4, 8>(nic, njc, ldi, rec, b, z, transpose, s);
-    else if (w == 16) cilu_apply_t<4, 16>(nic, njc, ldi, rec, b, z, transpose, s);
-    else cilu_apply_t<4, 32>(nic, njc, ldi, rec, b, z, transpose, s);
+    if (w == 8) cilu_apply_t<4, 8>(nic, njc, ldi, store, b, z, transpose, s);
+    else if (w == 16) cilu_apply_t<4, 16>(nic, njc, ldi, store, b, z, transpose, s);
+    else cilu_apply_t<4, 32>(nic, njc, ldi, store, b, z, transpose, s);
   }
   return hipGetLastError();
 }
