@@ -129,14 +129,25 @@ def ilu_mc5(B, idt, v):
     return z
 
 
-def _ilu_ordered(B, idt, v, order, keep, transpose=False, f32=False):
+def f16_scaled(a):
+    """A block stored in float16 after scaling by its largest magnitude (the
+    scale kept exactly), read back as float64."""
+    s = np.abs(a).max()
+    if s == 0.0:
+        return a
+    return (a / s).astype(np.float16).astype(np.float64) * s
+
+
+def _ilu_ordered(B, idt, v, order, keep, transpose=False, f32=False, rnd=None):
     """Block ILU(0) of I/dt - J on the 13-point stencil with the cells in the
     given order, keeping only the couplings keep((i,j), (ii,jj)) accepts.
     f32: the factors (L, U and the inverted diagonal blocks) are stored
     rounded to float32 and later rows and the solves use the stored values,
     as the device factorisation does; a row's own L blocks enter its upper
-    updates unrounded."""
-    rnd = (lambda a: a.astype(np.float32).astype(np.float64)) if f32 else (lambda a: a)
+    updates unrounded.  rnd: another storage rounding of a block (overrides
+    f32)."""
+    if rnd is None:
+        rnd = (lambda a: a.astype(np.float32).astype(np.float64)) if f32 else (lambda a: a)
     v = np.asarray(v, dtype=np.float64)
     nic, njc, nv = v.shape
     pos = {c: n for n, c in enumerate(order)}
@@ -208,10 +219,17 @@ def ilu_natural(B, idt, v, transpose=False):
     return _ilu_ordered(B, idt, v, order, lambda a, b: True, transpose)
 
 
-def ilu_chunked(B, idt, v, w, transpose=False, f32=True):
+def ilu_chunked(B, idt, v, w, transpose=False, f32=True, nine_point=False, f16=False):
     """Chunked line block ILU(0): i-chunks of w columns, each ordered column
     by column with j fastest; couplings between chunks dropped; factors
-    stored in float32 as on the device."""
+    stored in float32 as on the device.  nine_point: the pattern is the 3x3
+    box only (the +-2 slots, which hold only fill for a first-order
+    Jacobian, are dropped).  f16: factors stored as float16 with a scale per
+    block instead of float32."""
     nic, njc = np.asarray(v).shape[:2]
     order = [(i, j) for i in range(nic) for j in range(njc)]  # chunks are contiguous in i
-    return _ilu_ordered(B, idt, v, order, lambda a, b: a[0] // w == b[0] // w, transpose, f32)
+    if nine_point:
+        keep = lambda a, b: a[0] // w == b[0] // w and abs(a[0] - b[0]) <= 1 and abs(a[1] - b[1]) <= 1
+    else:
+        keep = lambda a, b: a[0] // w == b[0] // w
+    return _ilu_ordered(B, idt, v, order, keep, transpose, f32, rnd=f16_scaled if f16 else None)

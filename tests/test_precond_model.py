@@ -176,3 +176,38 @@ def test_natural_order_ilu_is_exact_on_one_row():
         ref = np.linalg.solve(A.T if tr else A, v.reshape(-1)).reshape(v.shape)
         z = pm.ilu_natural(B, idt, v, transpose=tr)
         assert np.abs(z - ref).max() <= 1e-12 * np.abs(ref).max(), tr
+
+
+def test_nine_point_chunked_ilu_is_exact_on_one_tridiagonal_column():
+    """The 3x3-box pattern (nine_point): on one column whose J holds only
+    the j +- 1 couplings, the matrix is block tridiagonal, its LU has no
+    fill and the model is A^-1; with the j +- 2 couplings present it is not
+    (they are outside the kept pattern)."""
+    nic, njc, nv = 1, 12, 5
+    B, idt = _random_blocks(nic, njc, nv, seed=17)
+    v = np.random.default_rng(18).standard_normal((nic, njc, nv))
+    Bt = _keep_slots(B, [0, 1, 2])
+    for tr in (False, True):
+        A = _dense(Bt, idt)
+        ref = np.linalg.solve(A.T if tr else A, v.reshape(-1)).reshape(v.shape)
+        z = pm.ilu_chunked(Bt, idt, v, 8, transpose=tr, f32=False, nine_point=True)
+        assert np.abs(z - ref).max() <= 1e-12 * np.abs(ref).max(), tr
+    full = _exact(B, idt, v)
+    z = pm.ilu_chunked(B, idt, v, 8, f32=False, nine_point=True)
+    assert np.abs(z - full).max() > 1e-4 * np.abs(full).max()
+
+
+def test_float16_scaled_factors_round_at_float16_level():
+    """float16 storage with a scale per block moves the result by about
+    float16 rounding (2^-11 relative per entry), far more than float32."""
+    nic, njc, nv = 3, 10, 5
+    B, idt = _random_blocks(nic, njc, nv, seed=19)
+    v = np.random.default_rng(20).standard_normal((nic, njc, nv))
+    z64 = pm.ilu_chunked(B, idt, v, 8, f32=False)
+    z16 = pm.ilu_chunked(B, idt, v, 8, f16=True)
+    z32 = pm.ilu_chunked(B, idt, v, 8, f32=True)
+    e16 = np.abs(z16 - z64).max() / np.abs(z64).max()
+    e32 = np.abs(z32 - z64).max() / np.abs(z64).max()
+    assert 1e-6 < e16 < 1e-2 and e16 > 100 * e32
+    blk = np.array([[3.0, -1e-3], [2.5e4, 0.0]])
+    assert np.abs(pm.f16_scaled(blk) - blk).max() <= 2.0 ** -11 * 2.5e4
