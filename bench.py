@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--closure-grid", default="1409x641")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="MB: also time a plain device copy of this size (HBM counter calibration)")
+    ap.add_argument("--no-extra-grids", action="store_true",
+                    help="skip the 1409x641 roofline line and the 97x65 laminar evaluations/s")
     return ap.parse_args()
 
 
@@ -137,6 +139,81 @@ def closure_leg(grid):
     return out
 
 
+def time_residual(eq, q0, steps, min_bytes, with_norm, rotate=True):
+    """Average per-launch time (ms) of back-to-back residual launches on the
+    library's stream, timed with HIP events on that stream.  rotate: cycle
+    over independent state/residual sets of at least min_bytes in all (cold,
+    beyond the 256 MB Infinity Cache); else one set (hot)."""
+    import torch
+    dev = torch.device("cuda:0")
+    nv, plane = eq.nvar, eq.plane
+    set_bytes = 2 * nv * plane * 8
+    nsets = max(2, int(np.ceil(min_bytes / set_bytes))) if rotate else 1
+    aos = torch.from_numpy(q0.reshape(-1)).to(dev)
+    qs, rs = [], []
+    for _ in range(nsets):
+        qd = torch.zeros(nv * plane, dtype=torch.float64, device=dev)
+        eq.aos_to_soa_dev(aos.data_ptr(), qd.data_ptr())
+        qs.append(qd)
+        rs.append(torch.zeros(nv * plane, dtype=torch.float64, device=dev))
+    l2 = torch.zeros(nv, dtype=torch.float64, device=dev)
+    eq.synchronize()
+    stream = torch.cuda.ExternalStream(eq.stream(), device=dev)
+
+    def run(n):
+        for k in range(n):
+            s = k % nsets
+            eq.residual_dev_async(qs[s].data_ptr(), rs[s].data_ptr(), l2.data_ptr() if with_norm else None)
+
+    run(min(steps, 20))
+    eq.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    run(steps)
+    e1.record(stream)
+    eq.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    l2h = l2.cpu().numpy()
+    del qs, rs
+    torch.cuda.empty_cache()
+    return ms, nsets * set_bytes, l2h
+
+
+def extra_grids(steps):
+    """The 1409 x 641 bump (h 0.065, w 0.25, Re 5e6; the largest grid of the
+    workloads) residual roofline, cold and hot, and the 97 x 65 laminar
+    plate's residual evaluations per second (whole evaluations: kernel and
+    norms)."""
+    from structured_amd import cases, lib
+    out = {}
+    case = cases.sa_bump(1409, 641, height=0.065, width=0.25, re=5e6)
+    eq = lib.Equation(case)
+    q0 = case.perturbed_state(seed=3)
+    cells = case.nic * case.njc
+    n = max(20, steps // 4)
+    ms_cold, nbytes, _ = time_residual(eq, q0, n, 1.2e9, with_norm=False)
+    ms_hot, _, _ = time_residual(eq, q0, n, 0, with_norm=False, rotate=False)
+    ms_step, _, _ = time_residual(eq, q0, n, 1.2e9, with_norm=True)
+    eq.close()
+    ach = 136 * cells / (ms_cold * 1e-3)
+    out["bump_1409x641"] = {"cells": cells, "kernel_ms_cold": ms_cold, "kernel_ms_hot": ms_hot,
+                            "step_ms_cold_with_norm": ms_step, "rotating_bytes": nbytes,
+                            "mcells_s_step": cells / (ms_step * 1e-3) / 1e6,
+                            "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9,
+                                         "unit": "GB/s", "frac": ach / HBM_PEAK, "bytes_per_cell": 136}}
+    case = cases.laminar_plate(97, 65)
+    eq = lib.Equation(case)
+    q0 = case.perturbed_state(seed=3)
+    cells = case.nic * case.njc
+    ms_step, nbytes, _ = time_residual(eq, q0, steps, 600e6, with_norm=True)
+    ms_hot, _, _ = time_residual(eq, q0, steps, 0, with_norm=True, rotate=False)
+    eq.close()
+    out["laminar_97x65"] = {"cells": cells, "residual_evals_per_s_cold": 1e3 / ms_step,
+                            "residual_evals_per_s_hot": 1e3 / ms_hot, "step_ms_cold": ms_step, "step_ms_hot": ms_hot,
+                            "mcells_s_cold": cells / (ms_step * 1e-3) / 1e6, "bytes_per_cell": 104}
+    return out
+
+
 def measured_traffic(ni, nj):
     """HBM bytes per launch of the residual kernel from the committed
     counter profile (FETCH_SIZE / WRITE_SIZE passes scaled by a copy of known
@@ -192,7 +269,7 @@ def main():
         eq.aos_to_soa_dev(aos.data_ptr(), qd.data_ptr())
         qs.append(qd)
         rs.append(torch.zeros(nv * plane, dtype=torch.float64, device=dev))
-    l2 = torch.zeros(8, dtype=torch.float64, device=dev)
+    l2 = torch.zeros(nv, dtype=torch.float64, device=dev)  # sg_residual_dev_async writes nvar norms
     eq.synchronize()
     stream = torch.cuda.ExternalStream(eq.stream(), device=dev)
 
@@ -226,18 +303,26 @@ def main():
         raise SystemExit("non-finite residual norm")
     bytes_cell = 17 * 8
     achieved = bytes_cell * cells / (ms_kernel * 1e-3)
-    # tangent (J.v) throughput on the same grid, one buffer set
+    # hot: the same launches on one buffer set (working set in the caches)
+    h0, h1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    h0.record(stream)
+    for _ in range(a.steps):
+        eq.residual_dev_async(qs[0].data_ptr(), rs[0].data_ptr(), l2.data_ptr())
+    h1.record(stream)
+    eq.synchronize()
+    ms_step_hot = h0.elapsed_time(h1) / a.steps
+    # tangent (J.v) kernel, back to back without synchronising, rotating over
+    # the state sets (the direction and output are one extra pair)
     vd = torch.randn(nv * plane, dtype=torch.float64, device=dev)
     jv_out = torch.zeros_like(vd)
     for _ in range(5):
-        eq.jv_dev(qs[0].data_ptr(), vd.data_ptr(), jv_out.data_ptr())
+        eq.jv_dev_async(qs[0].data_ptr(), vd.data_ptr(), jv_out.data_ptr())
+    eq.synchronize()
     j0, j1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     nj = max(10, a.steps // 10)
     j0.record(stream)
-    lib_ = lib.load()
-    import ctypes as C
-    for _ in range(nj):
-        lib_.sg_jv_dev(eq.h, C.c_void_p(qs[0].data_ptr()), C.c_void_p(vd.data_ptr()), C.c_void_p(jv_out.data_ptr()))
+    for k in range(nj):
+        eq.jv_dev_async(qs[k % nsets].data_ptr(), vd.data_ptr(), rs[k % nsets].data_ptr())
     j1.record(stream)
     eq.synchronize()
     ms_jv = j0.elapsed_time(j1) / nj
@@ -262,8 +347,10 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": measured_traffic(a.ni, a.nj),
                      "kernel": "residual_kernel<double,SA>",
                      "kernel_ms": ms_kernel, "bytes_per_cell": bytes_cell},
+        "hot": {"ms_per_step": ms_step_hot, "mcells_s": cells / (ms_step_hot * 1e-3) / 1e6},
         "jv": {"ms": ms_jv, "mcells_s": cells / (ms_jv * 1e-3) / 1e6,
-               "achieved_GBs_at_176B_cell": 176 * cells / (ms_jv * 1e-3) / 1e9},
+               "achieved_GBs_at_176B_cell": 176 * cells / (ms_jv * 1e-3) / 1e9,
+               "frac": 176 * cells / (ms_jv * 1e-3) / HBM_PEAK},
         "wall_s": wall,
         "l2": [float(x) for x in l2h],
     }
@@ -285,6 +372,10 @@ def main():
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, q0, a.cpu_seconds)
     eq.close()
+    del qs, rs
+    torch.cuda.empty_cache()
+    if not a.no_extra_grids:
+        out["grids"] = extra_grids(a.steps)
     if not a.no_adjoint:
         out["adjoint_gradient"] = adjoint_leg(a.adjoint_grid)
     if not a.no_closure:

@@ -107,6 +107,13 @@ int sg_get_metrics(sg_ctx* ctx, double* normal_chi, double* normal_eta, double* 
  *      L2 norm of Solver::step, src/solver/solver.cpp:125-134.  l2 may be NULL. */
 int sg_residual(sg_ctx* ctx, const double* q_aos, double* r_aos, double* l2);
 
+/* ---- initial state: EulerEquation::initialize, src/model/eulerequation.cpp:262-277.
+ *      Fills q_aos [nic][njc][nvar] with the freestream conservative state
+ *      (rho, rho u, rho v, p/(gamma-1) + rho |u|^2 / 2) and, with the SA
+ *      model, rho nu~_inf as the fifth unknown (the ntrans hook of
+ *      src/solver/solution.cpp:9). */
+int sg_initialize(sg_ctx* ctx, double* q_aos);
+
 /* ---- tangent: J.v with J = dR/dq at q (replaces the ADOL-C tape +
  *      sparse_jac forward sweep, src/solver/solver.cpp:69-90,153-158). */
 int sg_jv(sg_ctx* ctx, const double* q_aos, const double* v_aos, double* jv_aos);
@@ -179,9 +186,12 @@ int sg_write_tecplot(sg_ctx* ctx, const char* path, const double* q_aos);
 /* ---- device-resident variants (SoA device pointers, see header comment) */
 int sg_residual_dev(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_host);
 int sg_jv_dev(sg_ctx* ctx, const double* q_dev, const double* v_dev, double* jv_dev);
+/* sg_jv_dev without synchronising (enqueued on the context's stream). */
+int sg_jv_dev_async(sg_ctx* ctx, const double* q_dev, const double* v_dev, double* jv_dev);
 int sg_jtv_dev(sg_ctx* ctx, const double* q_dev, const double* psi_dev, double* jtpsi_dev, double* dbeta_dev);
 /* Enqueue the residual (and, if l2_dev is not NULL, its per-equation L2 norm
- * into device memory) without synchronising. */
+ * ||R_k||_2 into l2_dev[0..nvar-1], device memory: exactly nvar doubles are
+ * written) without synchronising. */
 int sg_residual_dev_async(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_dev);
 int sg_synchronize(sg_ctx* ctx);
 /* The context's HIP stream (a hipStream_t), so callers can order their own
@@ -215,9 +225,13 @@ int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos, int tra
 /* ---- nonlinear solve: Solver::solve / Solver::step, src/solver/solver.cpp:52-236.
  *      Ramp semantics as the reference (after step `counter`, if
  *      counter > ramp_iteration: cfl = min(cfl^exponent, cfl_max), UR =
- *      min(UR^exponent, 10)); ser > 0 instead sets cfl = cfl0 *
- *      (|R0|/|R|)^ser_exponent.  Stops when every equation's L2 norm is below
- *      tolerance (SG_OK) or after iteration_max steps (SG_ERR_NOCONV). */
+ *      min(UR^exponent, 10), solver.cpp:200-230); ser > 0 instead sets cfl =
+ *      cfl0 * (|R0|/|R|)^ser_exponent.  Stops when every equation's L2 norm
+ *      is below tolerance (SG_OK), or, like Solver::step (solver.cpp:136),
+ *      at the residual evaluation whose counter exceeds iteration_max, before
+ *      its linear solve: counters 0..iteration_max each solve, so
+ *      iteration_max + 1 linear solves (SG_ERR_NOCONV).  report->iterations
+ *      is the number of linear solves (Newton updates) made. */
 typedef struct sg_newton_opts {
   double cfl;
   int cfl_ramp;
@@ -269,9 +283,15 @@ typedef struct sg_solve_report {
 } sg_solve_report;
 
 int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* opts, sg_solve_report* report);
-/* Per-iteration L2 norms (nvar per entry) and CFL of the last sg_solve;
- * returns the number of entries recorded. */
+/* Per-iteration L2 norms (nvar per entry) and CFL of the last sg_solve, one
+ * entry per residual evaluation (report->iterations + 1 of them); returns the
+ * number of entries recorded. */
 int sg_solve_history(sg_ctx* ctx, double* l2_hist, double* cfl_hist, int max_entries);
+/* The CFL and the under-relaxation factor in force at each residual
+ * evaluation of the last sg_solve (the CFL, UNDER_RELAXATION pair that
+ * Solver::step uses for that counter, src/solver/solver.cpp:63,173); returns
+ * the number of entries recorded. */
+int sg_solve_ramp_history(sg_ctx* ctx, double* cfl_hist, double* ur_hist, int max_entries);
 
 /* ---- explicit pseudo-time stepping: the build of Solver::step without
  *      ADOL-C, src/solver/solver.cpp:103-121, with the update kernels

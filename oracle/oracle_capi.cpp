@@ -105,6 +105,29 @@ int og_residual(int ni, int nj, const double* xv, const double* yv, const sg_fre
   return 0;
 }
 
+// Ghosted primitives after primvars and the boundary list (src/model/fluid.cpp:50-67,
+// src/model/bc.cpp:430-433): out[6][nic+2][njc+2] = rho, u, v, p, T, nu~
+// (nu~ zero for the laminar model); cells no boundary writes are NaN.
+int og_ghosts(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs, int nbc,
+              const sg_opts* o, const double* q, double* out) {
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  Residual<double> res;
+  init_res(res, s, nullptr, nullptr);
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (Arr2<double>* a : {&res.rho, &res.u, &res.v, &res.p, &res.Tt, &res.nt}) std::fill(a->a.begin(), a->a.end(), nan);
+  res.primvars(q);
+  res.apply_bcs();
+  const size_t m = (s.geo.nic + 2) * (s.geo.njc + 2);
+  int k = 0;
+  for (Arr2<double>* a : {&res.rho, &res.u, &res.v, &res.p, &res.Tt, &res.nt}) {
+    if (a == &res.nt && !s.prm.sa) std::fill(out + k * m, out + (k + 1) * m, 0.0);
+    else std::copy(a->a.begin(), a->a.end(), out + k * m);
+    ++k;
+  }
+  return 0;
+}
+
 // Features and beta per cell at q: out[nic][njc][6].
 int og_closure_eval(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs,
                     int nbc, const sg_opts* o, const double* wd, const double* beta, const double* q, double* out) {

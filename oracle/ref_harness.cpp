@@ -6,6 +6,12 @@
 //   ref_harness <config.inp> res  <state.bin> <rhs.bin>
 //   ref_harness <config.inp> dt   <state.bin> <cfl> <dt.bin>
 //   ref_harness <config.inp> surf <state.bin>          (writes <label>.surface)
+//   ref_harness <config.inp> forces <state.bin>  (as surf, with the console at
+//              info level so the reference prints its Pressure / Viscous /
+//              Total Cl Cd lines, src/utils/io.cpp:249-251, to stdout)
+//   ref_harness <config.inp> init <state.bin> <out.bin>  (the state q that
+//              Mesh::setup leaves after EulerEquation::initialize,
+//              src/model/eulerequation.cpp:262-291; state.bin is not read)
 //   ref_harness <config.inp> grad <state.bin> <out.bin> (grad_{u,v,T}_{chi,eta})
 //   ref_harness <config.inp> solve <state.bin> <out.bin> (Solver::solve from the
 //              state with the config's explicit scheme; writes the final q)
@@ -49,6 +55,10 @@ int main(int argc, char** argv) {
   const size_t nic = mesh->nic, njc = mesh->njc, nv = sol->nq + sol->ntrans;
   const size_t n = nic * njc * nv;
   std::string mode = argv[2];
+  if (mode == "init") {
+    write_bin(argv[4], sol->q.data(), n);
+    return 0;
+  }
   std::vector<double> q = read_bin(argv[3], n);
   for (size_t m = 0; m < n; ++m) sol->q.data()[m] = q[m];
   if (mode == "res") {
@@ -60,6 +70,11 @@ int main(int argc, char** argv) {
   } else if (mode == "surf") {
     mesh->equation->calc_residual(sol->q.const_ref(), sol->rhs);
     mesh->fluid_model->primvars(sol->q.const_ref(), sol->rho, sol->u, sol->v, sol->p, sol->T);
+    mesh->iomanager->write_surface();
+  } else if (mode == "forces") {
+    mesh->equation->calc_residual(sol->q.const_ref(), sol->rhs);
+    mesh->fluid_model->primvars(sol->q.const_ref(), sol->rho, sol->u, sol->v, sol->p, sol->T);
+    logger->set_level(spdlog::level::info);
     mesh->iomanager->write_surface();
   } else if (mode == "grad") {
     mesh->equation->calc_residual(sol->q.const_ref(), sol->rhs);

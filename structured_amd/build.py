@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libstructured_gpu.so")
-SOURCES = ["sg_api.cpp", "sg_kernels.hip", "sg_linalg.hip", "sg_adjoint.hip", "sg_solver.cpp", "sg_mlp.hip",
+SOURCES = ["sg_api.cpp", "sg_kernels.hip", "sg_linalg.hip", "sg_solver.cpp", "sg_mlp.hip",
            "sg_mlp_api.cpp", "sg_inversion.cpp", "sg_cilu.hip", "sg_explicit.hip",
            "sg_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

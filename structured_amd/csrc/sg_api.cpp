@@ -543,8 +543,9 @@ int sg_get_metrics(sg_ctx* ctx, double* nchi, double* neta, double* volume, doub
 
 int sg_residual_dev_async(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_dev) {
   if (!ctx || !q_dev || !r_dev) return SG_ERR_ARG;
+  // the caller's l2_dev holds nvar norms (||R_k||_2): only those are written
   SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, l2_dev ? ctx->partial.p : nullptr,
-                         reinterpret_cast<unsigned*>(ctx->counter.p), l2_dev, ctx->stream, ctx->clo),
+                         reinterpret_cast<unsigned*>(ctx->counter.p), l2_dev, ctx->stream, ctx->clo, ctx->nvar),
          "residual launch");
   return SG_OK;
 }
@@ -576,6 +577,34 @@ int sg_residual(sg_ctx* ctx, const double* q_aos, double* r_aos, double* l2) {
   SG_TRY(hipStreamSynchronize(ctx->stream), "residual");
   if (l2) std::copy(l2h, l2h + ctx->nvar, l2);
   return check_finite(ctx, l2h);
+}
+
+int sg_jv_dev_async(sg_ctx* ctx, const double* q_dev, const double* v_dev, double* jv_dev) {
+  if (!ctx || !q_dev || !v_dev || !jv_dev) return SG_ERR_ARG;
+  SG_TRY(launch_jv(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, v_dev, nullptr, jv_dev, ctx->stream, nullptr, ctx->clo),
+         "jv launch");
+  return SG_OK;
+}
+
+// EulerEquation::initialize, src/model/eulerequation.cpp:262-277: the uniform
+// freestream state (rho, rho u, rho v, p/(gamma-1) + rho |u|^2 / 2), with
+// rho nu~_inf as the fifth unknown of the SA model (the reference has no
+// transport equation; its ntrans hook is src/solver/solution.cpp:9).
+int sg_initialize(sg_ctx* ctx, double* q_aos) {
+  if (!ctx || !q_aos) return SG_ERR_ARG;
+  const sg_freestream& f = ctx->fs;
+  const double e = f.p_inf / (kGamma - 1.0) + 0.5 * f.rho_inf * (f.u_inf * f.u_inf + f.v_inf * f.v_inf);
+  const int nv = ctx->nvar;
+  const size_t nc = ctx->ncell();
+  for (size_t c = 0; c < nc; ++c) {
+    double* q = q_aos + c * nv;
+    q[0] = f.rho_inf;
+    q[1] = f.rho_inf * f.u_inf;
+    q[2] = f.rho_inf * f.v_inf;
+    q[3] = e;
+    if (nv > 4) q[4] = f.rho_inf * f.nut_inf;
+  }
+  return SG_OK;
 }
 
 int sg_jv_dev(sg_ctx* ctx, const double* q_dev, const double* v_dev, double* jv_dev) {
@@ -659,6 +688,6 @@ int sg_soa_to_aos_dev(sg_ctx* ctx, const double* soa_dev, double* aos_dev, int n
   return SG_OK;
 }
 
-// Adjoint entry points are implemented in sg_adjoint.hip.
+// The adjoint entry points (sg_jtv, sg_adjoint_*, sg_objective_cf) are in sg_solver.cpp.
 
 }  // extern "C"

@@ -353,7 +353,8 @@ __device__ __forceinline__ SaTerms<S> sa_terms(S r, S n, S ml, const S* gu, cons
 // Fused norm output: with l2 set, the last block finalises the norms.
 struct NormOut {
   unsigned* counter;  // arrival count, 0 between launches
-  double* l2;         // 2 NV norms: ||R_k||_2, then ||V R_k||_2
+  double* l2;         // ||R_k||_2 (k < NV), then ||V R_k||_2 (NV <= k < 2 NV)
+  int count;          // entries of l2 written: NV (caller's buffer) or 2 NV
 };
 
 // CL: closure kind of beta (ClosureKind); CLOSURE_MLP stages the hidden
@@ -732,7 +733,7 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
           for (int k = 0; k < 2 * NV; ++k) red[wv][k] = acc[k];
         }
         __syncthreads();
-        if (tid < 2 * NV) {
+        if (tid < 2 * NV && tid < nout.count) {
           double s = 0.0;
           for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
           nout.l2[tid] = sqrt(s);

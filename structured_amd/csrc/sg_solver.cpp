@@ -44,7 +44,13 @@ struct SolverState {
   DevBuf blocks_pc;          // preconditioner blocks when lhs_order > 0 (blocks keeps the exact J)
   DevBuf cilu;               // chunked line ILU(0) factors (precond 6)
   int pc_width = 16;         // chunk width of precond 6
+  // have_lhs: the operator and preconditioner of the caller's last set_lhs are
+  // in place.  Every call that changes the preconditioner settings or
+  // overwrites the linearisation state (preallocate, set_pc, sg_solve, the
+  // adjoint paths) clears it, so solve / precond_apply never apply factors
+  // built for other settings or never built at all.
   bool have_lhs = false, have_rhs = false;
+  int fact_precond = -1, fact_width = 0;  // settings the current factors were built with
   int precond = 5;           // 0 none, 1 block Jacobi, 2 j-line, 3 line Gauss-Seidel, 4 multicolour ILU(0),
                              // 5 level-scheduled natural-order ILU(0), 6 chunked line ILU(0)
   int sweeps = 1;
@@ -55,7 +61,7 @@ struct SolverState {
   int maxit = 60;
   int last_iters = 0;
   double last_relres = 0.0;
-  std::vector<double> l2_hist, cfl_hist;
+  std::vector<double> l2_hist, cfl_hist, ur_hist;
   int nhist = 0;
 };
 
@@ -214,10 +220,14 @@ int extract_pc_blocks(sg_ctx* ctx) {
 
 int build_preconditioner(sg_ctx* ctx) {
   SolverState* S = ctx->solver;
-  if (S->precond == 0) return SG_OK;
+  S->fact_precond = -1;
+  if (S->precond == 0) { S->fact_precond = 0; return SG_OK; }
   int rc;
   if ((rc = extract_pc_blocks(ctx))) return rc;
-  return factor_preconditioner(ctx, S->idt.p);
+  if ((rc = factor_preconditioner(ctx, S->idt.p))) return rc;
+  S->fact_precond = S->precond;
+  S->fact_width = S->pc_width;
+  return SG_OK;
 }
 
 // Right-preconditioned restarted FGMRES with CGS2.  Solves A x = rhs.
@@ -336,6 +346,7 @@ int sg_ls_preallocate(sg_ctx* ctx, int restart, int precond, int maxit, double r
   if (!ctx || restart < 1 || restart > 200 || precond < 0 || precond > 6) return SG_ERR_ARG;
   int rc = ensure(ctx, restart);
   if (rc) return rc;
+  ctx->solver->have_lhs = false;
   ctx->solver->precond = precond;
   ctx->solver->pc_boost = 1.0;
   ctx->solver->maxit = maxit > 0 ? maxit : restart;
@@ -349,6 +360,7 @@ int sg_ls_set_pc(sg_ctx* ctx, int lhs_order, double pc_dt_factor, int pc_width) 
   ctx->solver->lhs_order = lhs_order;
   ctx->solver->pc_boost = pc_dt_factor;
   if (pc_width) ctx->solver->pc_width = pc_width;
+  ctx->solver->have_lhs = false;  // the factors of the last set_lhs no longer match
   return SG_OK;
 }
 
@@ -379,7 +391,9 @@ int sg_ls_set_rhs(sg_ctx* ctx, const double* rhs_aos) {
 int sg_ls_solve(sg_ctx* ctx, double* dq_aos, int* iters, double* relres) {
   if (!ctx || !ctx->solver) return SG_ERR_ARG;
   SolverState* S = ctx->solver;
-  if (!S->have_lhs || !S->have_rhs) return sg_fail(ctx, SG_ERR_STATE, "set_lhs and set_rhs must precede solve");
+  if (!S->have_lhs || !S->have_rhs)
+    return sg_fail(ctx, SG_ERR_STATE, "set_lhs (after the last preallocate / set_pc / solve / adjoint call) and set_rhs "
+                                      "must precede solve");
   int it = 0, rc;
   double rr = 0.0;
   if ((rc = fgmres(ctx, S->rtol, S->maxit, &it, &rr))) return rc;
@@ -418,6 +432,7 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
   double cfl = o->cfl, ur = o->under_relaxation;
   S->l2_hist.clear();
   S->cfl_hist.clear();
+  S->ur_hist.clear();
   sg_solve_report R{};
   double l2[16], l2_0[16];
   int lin_total = 0;
@@ -439,6 +454,7 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     if (it == 0) std::copy(l2, l2 + nv, l2_0);
     for (int k = 0; k < nv; ++k) S->l2_hist.push_back(l2[k]);
     S->cfl_hist.push_back(cfl);
+    S->ur_hist.push_back(ur);
     bool finite = true;
     double lmax = 0.0;
     for (int k = 0; k < nv; ++k) {
@@ -457,7 +473,10 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
       continue;
     }
     if (lmax < o->tolerance) { converged = true; break; }
-    if (it >= o->iteration_max) break;
+    // Solver::step returns before the linear solve once counter >
+    // iteration_max (src/solver/solver.cpp:136): counters 0..iteration_max
+    // each solve, iteration_max + 1 linear solves in all
+    if (it > o->iteration_max) break;
     lmax_hist.push_back(lmax);
     if (o->stall_window > 0 && (int)lmax_hist.size() > 2 * o->stall_window) {
       const size_t w = (size_t)o->stall_window, m = lmax_hist.size();
@@ -539,8 +558,13 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
 
 // z = M^-1 v or M^-T v with the preconditioner of the last set_lhs.
 int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos, int transpose) {
-  if (!ctx || !v_aos || !z_aos || !ctx->solver || !ctx->solver->have_lhs) return SG_ERR_STATE;
+  if (!ctx || !v_aos || !z_aos) return SG_ERR_ARG;
+  if (!ctx->solver || !ctx->solver->have_lhs)
+    return sg_fail(ctx, SG_ERR_STATE, "set_lhs (after the last preallocate / set_pc / solve / adjoint call) must "
+                                      "precede precond_apply");
   SolverState* S = ctx->solver;
+  if (S->fact_precond != S->precond || (S->precond == 6 && (S->fact_width != S->pc_width || !S->cilu.p)))
+    return sg_fail(ctx, SG_ERR_STATE, "preconditioner factors do not match the current settings");
   int rc;
   if ((rc = upload(ctx, v_aos, S->tmp.p))) return rc;
   S->transpose = transpose != 0;
@@ -564,6 +588,8 @@ int adjoint_prepare(sg_ctx* ctx, const double* q_dev) {
   int rc;
   if (!ctx->solver && (rc = ensure(ctx, 30))) return rc;
   SolverState* S = ctx->solver;
+  S->have_lhs = false;  // qop and the Jacobian blocks are overwritten below
+  S->fact_precond = -1;
   SG_TRY(hipMemcpyAsync(S->qop.p, q_dev, S->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
   if ((rc = extract_jacobian(ctx))) return rc;
   if (ctx->sa) {
@@ -737,6 +763,18 @@ int sg_adjoint_gradient(sg_ctx* ctx, const double* q_aos, const double* cf_targe
   for (int i = 0; i < G.nic; ++i)
     for (int j = 0; j < G.njc; ++j) grad_beta[(size_t)i * G.njc + j] = -h[(size_t)j * ctx->g.ldi + i];
   return rs;
+}
+
+int sg_solve_ramp_history(sg_ctx* ctx, double* cfl_hist, double* ur_hist, int max_entries) {
+  if (!ctx || !ctx->solver) return SG_ERR_ARG;
+  const SolverState* S = ctx->solver;
+  const int n = (int)S->cfl_hist.size();
+  const int m = std::min(n, max_entries);
+  for (int i = 0; i < m; ++i) {
+    if (cfl_hist) cfl_hist[i] = S->cfl_hist[i];
+    if (ur_hist) ur_hist[i] = S->ur_hist[i];
+  }
+  return n;
 }
 
 int sg_solve_history(sg_ctx* ctx, double* l2_hist, double* cfl_hist, int max_entries) {

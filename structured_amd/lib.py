@@ -233,6 +233,16 @@ class Equation:
     def jv_dev(self, q_ptr: int, v_ptr: int, out_ptr: int):
         self._check(load().sg_jv_dev(self.h, C.c_void_p(q_ptr), C.c_void_p(v_ptr), C.c_void_p(out_ptr)))
 
+    def jv_dev_async(self, q_ptr: int, v_ptr: int, out_ptr: int):
+        self._check(load().sg_jv_dev_async(self.h, C.c_void_p(q_ptr), C.c_void_p(v_ptr), C.c_void_p(out_ptr)))
+
+    def initialize(self):
+        """EulerEquation::initialize (src/model/eulerequation.cpp:262-277):
+        the freestream state [nic, njc, nvar] (rho nu~_inf fifth with SA)."""
+        q = np.empty(self.shape)
+        self._check(load().sg_initialize(self.h, _p(q)))
+        return q
+
     def synchronize(self):
         self._check(load().sg_synchronize(self.h))
 
@@ -256,12 +266,15 @@ class Equation:
         n = load().sg_solve_history(self.h, None, None, 0)
         l2h = np.zeros((max(n, 1), self.nvar))
         cfl = np.zeros(max(n, 1))
+        ur = np.zeros(max(n, 1))
         load().sg_solve_history(self.h, _p(l2h), _p(cfl), n)
+        load().sg_solve_ramp_history(self.h, None, _p(ur), n)
         report = {f: getattr(rep, f) for f, _ in abi.SolveReport._fields_}
         report["l2_initial"] = list(rep.l2_initial)[: self.nvar]
         report["l2_final"] = list(rep.l2_final)[: self.nvar]
         report["l2w_final"] = list(rep.l2w_final)[: self.nvar]
-        return q, report, {"l2": l2h[:n], "cfl": cfl[:n]}
+        report["code"] = rc
+        return q, report, {"l2": l2h[:n], "cfl": cfl[:n], "ur": ur[:n]}
 
     def step_explicit_dev(self, q_ptr: int, scheme="rk4_jameson", cfl=1.0):
         """One explicit step on a device SoA state in place; returns the

@@ -14,9 +14,12 @@ explicit_*.npz hold the reference's Solver::solve result for the explicit
 schemes (forward Euler, RK4-Jameson) from a fixture's state (explicit.json).
 naca0012_281x151.npz holds the reference's outputs on its own NACA 0012
 example grid (naca_fixture).
+ref_scalars.json holds the reference's printed force coefficients and its
+initial state per viscous fixture (ref_scalars).
 Run:  python tests/golden/make_golden.py [case names]
       python tests/golden/make_golden.py explicit
       python tests/golden/make_golden.py naca
+      python tests/golden/make_golden.py scalars
 """
 import json
 import os
@@ -192,8 +195,36 @@ def naca_fixture():
     print("naca0012_281x151 max|R| %.3e" % np.abs(R).max())
 
 
+def ref_scalars():
+    """ref_scalars.json: per viscous fixture (and the NACA 0012 example),
+    the reference's six force coefficients on the fixture's state as its
+    write_surface prints them (src/utils/io.cpp:239-251), and the state its
+    EulerEquation::initialize sets (one cell: every cell is the same,
+    checked here; src/model/eulerequation.cpp:262-277)."""
+    import golden_util as gu
+    out = {}
+    todo = [(n,) + gu.load(n)[:2] for n in gu.names()]
+    from structured_amd import lib
+    xv, yv = lib.read_grid(NACA_GRID, "simple", 281, 151)
+    naca = cases.naca0012_example(xv, yv)
+    todo.append(("naca0012_281x151", naca, {"q": naca.perturbed_state(seed=7)}))
+    for name, case, z in todo:
+        if case.fs["mu_inf"] <= 1e-15:
+            continue
+        ref = of.RefRun(case)
+        f = ref.forces(z["q"])
+        q0 = ref.init()
+        assert np.all(q0 == q0[0, 0]), name
+        out[name] = dict(forces=[float(x) for x in f], init=[float(x) for x in q0[0, 0]])
+        print(name, "forces", f)
+    with open(os.path.join(HERE, "ref_scalars.json"), "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["explicit"]:
+    if sys.argv[1:] == ["scalars"]:
+        ref_scalars()
+    elif sys.argv[1:] == ["explicit"]:
         explicit_fixtures()
     elif sys.argv[1:] == ["naca"]:
         naca_fixture()

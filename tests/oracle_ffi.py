@@ -102,6 +102,17 @@ def surface(case, q, wd=None, beta=None):
     return x, cp, cf, forces
 
 
+def ghosts(case, q):
+    """Ghosted primitives [6, nic+2, njc+2] (rho, u, v, p, T, nu~) after
+    primvars and the boundary list; cells no boundary writes are NaN."""
+    xv, yv, fs, bcs, o = _common(case)
+    q = _arr(q)
+    out = np.zeros((6, case.nic + 2, case.njc + 2))
+    lib().og_ghosts(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs), C.byref(o),
+                    abi.dptr(q), abi.dptr(out))
+    return out
+
+
 def wall_distance(case):
     xv, yv, fs, bcs, o = _common(case)
     d = np.zeros((case.nic, case.njc))
@@ -184,6 +195,30 @@ class RefRun:
     def surface(self, q):
         self._run("surf", self._state(q))
         return np.loadtxt(os.path.join(self.dir, "case.surface"))
+
+    def forces(self, q):
+        """The six force coefficients the reference prints after writing the
+        surface file (src/utils/io.cpp:239-251): [cl_p, cd_p, cl_v, cd_v, cl,
+        cd], parsed from its 'Pressure / Viscous / Total Cl: .. Cd: ..'
+        console lines (the bundled fmt prints each double with 6 significant
+        digits, as %g does)."""
+        r = subprocess.run([REF_HARNESS, "case.inp", "forces", self._state(q)], cwd=self.dir, check=True,
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+        import re
+        vals = {}
+        for line in re.sub(r"\x1b\[[0-9;]*m", "", r.stdout).splitlines():
+            parts = line.split()
+            if len(parts) == 5 and parts[1] == "Cl:" and parts[3] == "Cd:":
+                vals[parts[0]] = (float(parts[2]), float(parts[4]))
+        return np.array([*vals["Pressure"], *vals["Viscous"], *vals["Total"]])
+
+    def init(self):
+        """The state Mesh::setup leaves after EulerEquation::initialize
+        (src/model/eulerequation.cpp:262-291), [nic, njc, nvar]."""
+        c = self.case
+        shape = (c.nic, c.njc, 4)
+        self._run("init", self._state(np.zeros(shape)), "init.bin")
+        return np.fromfile(os.path.join(self.dir, "init.bin")).reshape(shape)
 
     def solve(self, q, **solver):
         """Solver::solve from q with the given [solver] keys (explicit
