@@ -21,9 +21,12 @@ adjoint_gradient: wall time of one adjoint beta-gradient of the wall-Cf
 objective on the 705 x 321 channel bump (beta = 1 + 8 seeded blobs, target Cf
 from a second seeded beta*), after converging the primal states.
 
-ml_closure_vs_sa: plain-SA solve against the solve with the closure network
-fused into the SA source, both from freestream on the 1409 x 641 bump
-(h 0.065, w 0.25, Re 5e6).
+ml_closure_vs_inversion: on the 1409 x 641 bump (h 0.065, w 0.25, Re 5e6),
+the solve with the closure network (trained on the inverted beta fields of the
+24 training bumps) fused into the SA source against the plain-SA solve and
+against a full adjoint field inversion on the same grid: wall-Cf L2 error
+of the closure solve against the inversion, and the inversion's time (its
+per-iteration time measured here) over the closure solve's.
 
 Prints one JSON line.
 """
@@ -58,6 +61,8 @@ def parse():
     ap.add_argument("--no-closure", action="store_true", help="skip the ML-closure vs plain-SA leg")
     ap.add_argument("--adjoint-grid", default="705x321")
     ap.add_argument("--closure-grid", default="1409x641")
+    ap.add_argument("--inversion-iterations", type=int, default=2,
+                    help="L-BFGS iterations of the field inversion timed in the closure leg")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="MB: also time a plain device copy of this size (HBM counter calibration)")
     ap.add_argument("--no-extra-grids", action="store_true",
@@ -70,7 +75,44 @@ def _grid(s):
     return int(ni), int(nj)
 
 
-def adjoint_leg(grid):
+def newton_leg(ni, nj):
+    """The SA flat plate's steady solve: Newton-GMRES(30) from freestream
+    with the fused residual and matrix-free J.v, chunked line ILU(0) of the
+    first-order Jacobian, to a flux-balance norm max_k ||V R_k||_2 < 1e-12.
+    Reports the reference's norm max_k ||R_k||_2 (R = flux balance / V) at
+    the end too: with 1e-6 wall spacing its FP64 roundoff floor lies far
+    above 1e-10, the flux-balance norm's does not."""
+    from structured_amd import cases, fiml, lib
+    case = cases.sa_plate(ni, nj)
+    eq = lib.Equation(case)
+    t0 = time.perf_counter()
+    q, r, hist = eq.solve(case.freestream_state(), **fiml.NEWTON)
+    t = time.perf_counter() - t0
+    eq.close()
+    return {"grid": "%dx%d" % (ni, nj), "solve_s": t, "newton": r["iterations"], "converged": bool(r["converged"]),
+            "linear_iterations": r["linear_iterations"], "l2w_final_max": max(r["l2w_final"]),
+            "l2_final_max": max(r["l2_final"]), "l2_initial_max": max(r["l2_initial"]),
+            "tolerance_l2w": fiml.NEWTON["tolerance"]}
+
+
+def adjoint_cpu_baseline(case, q, gmres_iterations):
+    """The adjoint gradient's CPU baseline on a bounded sample: the CPU
+    oracle's Jacobian-block extraction (5 x 5 coloured dual sweeps of the
+    restated reference residual) timed whole, plus 3 transposed Krylov
+    iterations (block J^T psi, block-Jacobi apply, CGS2 against 60 basis
+    vectors: the GPU run's restart 120 averages 60) timed and scaled to the
+    GPU run's iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as of
+    threads = os.cpu_count() or 1
+    t_blk, t_it = of.adjoint_sample(case, q, threads=threads, iters=3, basis=60)
+    return {"value": t_blk + gmres_iterations * t_it, "unit": "s", "cores": threads, "kind": "port",
+            "sample": "Jacobian blocks %.2f s (measured) + %d Krylov iterations x %.4f s (3 measured) on %d threads"
+                      % (t_blk, gmres_iterations, t_it, threads),
+            "jacobian_blocks_s": t_blk, "krylov_iteration_s": t_it}
+
+
+def adjoint_leg(grid, cpu=True):
     """Adjoint beta-gradient of the wall-Cf objective on the channel bump:
     beta = 1 + 8 seeded blobs, target Cf from a second seeded beta*.  The
     primal states are converged first (untimed for the metric); the metric is
@@ -101,41 +143,74 @@ def adjoint_leg(grid):
            "primal_warm_solve_s": t_warm, "primal_converged": [bool(r["converged"]) for r in (r1, r2, r3)],
            "inversion_iteration_s_estimate": t_warm + t_adj}
     eq.close()
+    if cpu:
+        out["cpu_baseline"] = adjoint_cpu_baseline(case, q, rep["iterations"])
     return out
 
 
-def closure_leg(grid):
-    """Plain-SA solve against the solve with the closure network fused into
-    the SA source (BF16 MFMA inference in every residual and J.v), both from
-    freestream on the same bump.  The network is trained on the GPU first
-    (untimed for the metric) on the features of the plain-SA solution against
-    a seeded analytic beta*."""
+DATA = os.path.join(ROOT, "structured_amd", "data")
+
+
+def closure_leg(grid, inversion_iterations):
+    """The learned closure in the solve loop against plain SA and against a
+    full adjoint field inversion, all on the unseen bump (h 0.065, w 0.25,
+    Re 5e6) on the same grid.
+
+    - The network (5-64-64-1 tanh) is the one trained on the GPU by this
+      library's kernels on the inverted beta fields of the 24 training bumps
+      (385x193, 50 L-BFGS iterations each): structured_amd/data/closure_net.npz.
+    - The inversion reference is structured_amd/data/reference_<grid>.npz: the
+      wall Cf of a 50-iteration sg_invert run on this grid against the
+      truth's Cf (beta* = 1 + 0.3 tanh(c . f + c0)), and its wall time.
+    - Timed here: the plain-SA solve and the closure solve (both from
+      freestream) and `inversion_iterations` L-BFGS iterations of the same
+      inversion from the plain-SA state, whose per-iteration time gives the
+      inversion's cost on this GPU."""
     from structured_amd import cases, fiml, lib
     ni, nj = _grid(grid)
+    net_path = os.path.join(DATA, "closure_net.npz")
+    ref_path = os.path.join(DATA, "reference_%s.npz" % grid)
     case = cases.sa_bump(ni, nj, height=0.065, width=0.25, re=5e6)
     eq = lib.Equation(case)
     t0 = time.perf_counter()
     q_sa, r_sa, _ = eq.solve(case.freestream_state(), **fiml.NEWTON)
     t_sa = time.perf_counter() - t0
-    coef = fiml.truth_coef(seed=1)
-    eq.set_closure(1, coef=coef)
-    feats, beta_t = eq.closure_eval(q_sa)
-    net, hist, t_train = fiml.train(feats.reshape(-1, 5).astype(np.float32),
-                                    (beta_t - 1.0).reshape(-1).astype(np.float32), epochs=20, batch=65536, lr=1e-3)
-    eq.set_closure_mlp(net)
+    _, _, cf_sa, _ = eq.wall(q_sa)
+    out = {"grid": grid, "cells": case.nic * case.njc, "plain_sa_solve_s": t_sa,
+           "plain_sa_newton": r_sa["iterations"], "plain_sa_converged": bool(r_sa["converged"])}
+    if not (os.path.exists(net_path) and os.path.exists(ref_path)):
+        out["missing"] = [p for p in (net_path, ref_path) if not os.path.exists(p)]
+        eq.close()
+        return out
+    params = np.load(net_path)["params"].astype(np.float32)
+    ref = np.load(ref_path)
+    cf_inv, cf_true = ref["cf_inv"], ref["cf_true"]
+    eq.set_closure(2, params=params)
     o = dict(fiml.NEWTON, tolerance=fiml.MLP_TOL, stall_window=fiml.MLP_STALL)
     t0 = time.perf_counter()
     q_ml, r_ml, _ = eq.solve(case.freestream_state(), **o)
     t_ml = time.perf_counter() - t0
-    _, _, cf_sa, _ = eq.wall(q_sa)
     _, _, cf_ml, _ = eq.wall(q_ml)
-    out = {"grid": grid, "cells": case.nic * case.njc, "plain_sa_solve_s": t_sa, "plain_sa_newton": r_sa["iterations"],
-           "plain_sa_converged": bool(r_sa["converged"]), "ml_closure_solve_s": t_ml,
-           "ml_closure_newton": r_ml["iterations"], "ml_closure_converged": bool(fiml.closure_solve_ok(r_ml)),
-           "ml_over_sa_time": t_ml / t_sa, "ml_l2w_final": max(r_ml["l2w_final"]),
-           "cf_change_vs_sa": fiml.cf_error(cf_ml, cf_sa), "train_s": t_train, "train_loss": [float(hist[0]),
-                                                                                               float(hist[-1])]}
+    eq.set_closure(0)
+    # the inversion's cost on this GPU: L-BFGS iterations from the plain-SA state
+    t0 = time.perf_counter()
+    _, _, irep, _ = eq.invert(q_sa, cf_true, iterations=inversion_iterations, newton=fiml.RESOLVE,
+                              adjoint=fiml.ADJOINT)
+    t_it = (time.perf_counter() - t0) / max(irep["iterations"], 1)
     eq.close()
+    n_ref = int(ref["iterations_done"])
+    out.update({
+        "ml_closure_solve_s": t_ml, "ml_closure_newton": r_ml["iterations"],
+        "ml_closure_converged": bool(fiml.closure_solve_ok(r_ml)), "ml_over_sa_time": t_ml / t_sa,
+        "cf_l2_err": fiml.cf_error(cf_ml, cf_inv),
+        "cf_l2_err_sa_vs_inversion": fiml.cf_error(cf_sa, cf_inv),
+        "cf_err_vs_truth": {"ml": fiml.cf_error(cf_ml, cf_true), "inversion": fiml.cf_error(cf_inv, cf_true),
+                            "plain_sa": fiml.cf_error(cf_sa, cf_true)},
+        "inversion_iterations_timed": int(irep["iterations"]), "inversion_iteration_s": t_it,
+        "inversion_reference_iterations": n_ref, "inversion_reference_s": float(ref["inversion_s"]),
+        "time_ratio": t_it * n_ref / t_ml,
+        "time_ratio_reference_run": float(ref["inversion_s"]) / t_ml,
+    })
     return out
 
 
@@ -376,10 +451,11 @@ def main():
     torch.cuda.empty_cache()
     if not a.no_extra_grids:
         out["grids"] = extra_grids(a.steps)
+        out["newton_sa_plate"] = newton_leg(a.ni, a.nj)
     if not a.no_adjoint:
-        out["adjoint_gradient"] = adjoint_leg(a.adjoint_grid)
+        out["adjoint_gradient"] = adjoint_leg(a.adjoint_grid, cpu=not a.no_cpu)
     if not a.no_closure:
-        out["ml_closure_vs_sa"] = closure_leg(a.closure_grid)
+        out["ml_closure_vs_inversion"] = closure_leg(a.closure_grid, a.inversion_iterations)
     print(json.dumps(out))
 
 

@@ -131,7 +131,8 @@ typedef struct sg_adjoint_opts {
   int maxit;       /* total Krylov iterations                               */
   double rtol;     /* relative residual tolerance                           */
   int precond;     /* 0 none, 4 five-colour ILU(0), 5 natural-order ILU(0),
-                      6 chunked line ILU(0)                                 */
+                      6 chunked line ILU(0) on the 13-point stencil, 7 the
+                      same on the 3x3-box pattern                           */
   int lhs_order;   /* 1: the preconditioner factors the first-order Jacobian */
   double pc_cfl;   /* the preconditioner factors I/dt - J at this CFL        */
 } sg_adjoint_opts;
@@ -205,7 +206,11 @@ void* sg_get_stream(sg_ctx* ctx);
  *      blocks per cell (precond 0: none, 1: block Jacobi, 2: j-line
  *      block-tridiagonal, 3: symmetric multicolour j-line Gauss-Seidel,
  *      4: five-colour block ILU(0) of I/dt - J on the 13-point stencil,
- *      5: block ILU(0) in natural j-major order, level-scheduled). 
+ *      5: block ILU(0) in natural j-major order, level-scheduled,
+ *      6: chunked line block ILU(0): chunks of pc_width columns, each ordered
+ *      column by column, couplings between chunks dropped, FP32 factors,
+ *      7: precond 6 on the 3x3-box pattern: the +-2 blocks and their fill
+ *      dropped, which with lhs_order 1 are fill only).
  *      The Krylov method is restarted FGMRES (restart length `restart`,
  *      at most `maxit` iterations, relative tolerance `rtol`). */
 int sg_ls_preallocate(sg_ctx* ctx, int restart, int precond, int maxit, double rtol);
@@ -219,7 +224,7 @@ int sg_ls_set_rhs(sg_ctx* ctx, const double* rhs_aos);
 int sg_ls_solve(sg_ctx* ctx, double* dq_aos, int* iters, double* relres);
 int sg_ls_solve_and_update(sg_ctx* ctx, double* q_aos, double under_relaxation);
 /* z = M^-1 v (transpose = 0) or M^-T v (transpose = 1, preconditioners 0,
- * 4, 5 and 6) with the preconditioner built by the last set_lhs. */
+ * 4, 5, 6 and 7) with the preconditioner built by the last set_lhs. */
 int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos, int transpose);
 
 /* ---- nonlinear solve: Solver::solve / Solver::step, src/solver/solver.cpp:52-236.

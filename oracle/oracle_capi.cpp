@@ -225,6 +225,129 @@ int og_wall_distance(int ni, int nj, const double* xv, const double* yv, const s
   return 0;
 }
 
+// CPU baseline of the adjoint gradient (a bounded sample, for the benchmark
+// only), on `threads` OpenMP threads:
+//   out[0]: seconds to extract the Jacobian's 13 stencil blocks per cell,
+//           by 5 x 5 cell colouring of dual sweeps of this residual (all nv
+//           directions of a colour in one sweep);
+//   out[1]: seconds per iteration of the transposed Krylov loop's work, the
+//           average of `iters` iterations: J^T psi gathered from the blocks,
+//           a block-Jacobi preconditioner apply and CGS2 orthogonalisation
+//           (two multi-dots and two multi-axpys) against `basis` vectors.
+int og_adjoint_sample(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs, const sg_bc* bcs,
+                      int nbc, const sg_opts* o, const double* q, int threads, int iters, int basis, double* out) {
+  using clk = std::chrono::steady_clock;
+  if (threads > 0) omp_set_num_threads(threads);
+  Setup s;
+  make_setup(s, ni, nj, xv, yv, fs, bcs, nbc, o);
+  const long nic = (long)s.geo.nic, njc = (long)s.geo.njc;
+  const int nv = s.prm.nvar();
+  const size_t nc = (size_t)nic * njc, n = nc * nv;
+  static const int DI[13] = {0, 0, 0, -1, 1, -1, 1, -1, 1, 0, 0, -2, 2};
+  static const int DJ[13] = {0, -1, 1, 0, 0, -1, -1, 1, 1, -2, 2, 0, 0};
+  // (1) blocks[c][s][k][m] = dR_(c,k) / dq_(c + off_s, m)
+  std::vector<double> blk(nc * 13 * nv * nv, 0.0);
+  auto t0 = clk::now();
+  {
+    Residual<Dual<5>> res;
+    init_res(res, s, nullptr, nullptr);
+    std::vector<Dual<5>> qd(n), rd(n);
+    for (int a = 0; a < 5; ++a)
+      for (int b = 0; b < 5; ++b) {
+        for (long i = 0; i < nic; ++i)
+          for (long j = 0; j < njc; ++j) {
+            const bool seed = i % 5 == a && j % 5 == b;
+            for (int m = 0; m < nv; ++m) {
+              Dual<5>& x = qd[((size_t)i * njc + j) * nv + m];
+              x = Dual<5>(q[((size_t)i * njc + j) * nv + m]);
+              if (seed) x.d[m] = 1.0;
+            }
+          }
+        res.evaluate(qd.data(), rd.data());
+#pragma omp parallel for
+        for (long i = 0; i < nic; ++i)
+          for (long j = 0; j < njc; ++j)
+            for (int sl = 0; sl < 13; ++sl) {
+              const long ii = i + DI[sl], jj = j + DJ[sl];
+              if (ii < 0 || ii >= nic || jj < 0 || jj >= njc || ii % 5 != a || jj % 5 != b) continue;
+              const size_t c = (size_t)i * njc + j;
+              for (int k = 0; k < nv; ++k)
+                for (int m = 0; m < nv; ++m) blk[((c * 13 + sl) * nv + k) * nv + m] = rd[c * nv + k].d[m];
+            }
+      }
+  }
+  out[0] = std::chrono::duration<double>(clk::now() - t0).count();
+  // block-Jacobi inverse of the diagonal blocks (set-up, not timed)
+  std::vector<double> dinv(nc * nv * nv);
+#pragma omp parallel for
+  for (long c = 0; c < (long)nc; ++c) {
+    double a[5][10];
+    for (int r = 0; r < nv; ++r)
+      for (int m = 0; m < 2 * nv; ++m) a[r][m] = m < nv ? -blk[(((size_t)c * 13) * nv + r) * nv + m] + (r == m ? 1e3 : 0.0)
+                                                        : (m - nv == r ? 1.0 : 0.0);
+    for (int p = 0; p < nv; ++p) {
+      const double ip = 1.0 / a[p][p];
+      for (int m = 0; m < 2 * nv; ++m) a[p][m] *= ip;
+      for (int r = 0; r < nv; ++r)
+        if (r != p) {
+          const double f = a[r][p];
+          for (int m = 0; m < 2 * nv; ++m) a[r][m] -= f * a[p][m];
+        }
+    }
+    for (int r = 0; r < nv; ++r)
+      for (int m = 0; m < nv; ++m) dinv[((size_t)c * nv + r) * nv + m] = a[r][nv + m];
+  }
+  // (2) Krylov iterations' work
+  const int nb = basis > 0 ? basis : 1;
+  std::vector<double> V((size_t)(nb + 1) * n), z(n), w(n), h(nb), h2(nb);
+  for (size_t m = 0; m < V.size(); ++m) V[m] = std::sin(0.001 * (double)m);
+  t0 = clk::now();
+  for (int it = 0; it < iters; ++it) {
+    const double* v = V.data() + (size_t)(it % nb) * n;
+#pragma omp parallel for
+    for (long c = 0; c < (long)nc; ++c)
+      for (int r = 0; r < nv; ++r) {
+        double acc = 0.0;
+        for (int m = 0; m < nv; ++m) acc += dinv[((size_t)c * nv + r) * nv + m] * v[(size_t)c * nv + m];
+        z[(size_t)c * nv + r] = acc;
+      }
+    // w = (J^T z)_col: the rows of column cell c are c - off_s, block s of that row
+#pragma omp parallel for
+    for (long i = 0; i < nic; ++i)
+      for (long j = 0; j < njc; ++j) {
+        const size_t c = (size_t)i * njc + j;
+        double acc[5] = {0, 0, 0, 0, 0};
+        for (int sl = 0; sl < 13; ++sl) {
+          const long ri = i - DI[sl], rj = j - DJ[sl];
+          if (ri < 0 || ri >= nic || rj < 0 || rj >= njc) continue;
+          const size_t r = (size_t)ri * njc + rj;
+          for (int k = 0; k < nv; ++k)
+            for (int m = 0; m < nv; ++m) acc[m] += blk[((r * 13 + sl) * nv + k) * nv + m] * z[r * nv + k];
+        }
+        for (int m = 0; m < nv; ++m) w[c * nv + m] = acc[m];
+      }
+    for (int pass = 0; pass < 2; ++pass) {
+      double* hh = pass ? h2.data() : h.data();
+      for (int b = 0; b < nb; ++b) {
+        const double* vb = V.data() + (size_t)b * n;
+        double acc = 0.0;
+#pragma omp parallel for reduction(+ : acc)
+        for (long m = 0; m < (long)n; ++m) acc += vb[m] * w[m];
+        hh[b] = acc;
+      }
+#pragma omp parallel for
+      for (long m = 0; m < (long)n; ++m) {
+        double acc = w[m];
+        for (int b = 0; b < nb; ++b) acc -= hh[b] * V[(size_t)b * n + m];
+        w[m] = acc;
+      }
+    }
+  }
+  out[1] = std::chrono::duration<double>(clk::now() - t0).count() / (iters > 0 ? iters : 1);
+  out[2] = w[n / 2] + h[0] + h2[0];  // keeps the work observable
+  return 0;
+}
+
 // CPU baseline timing: median wall time (seconds) of `reps` residual
 // evaluations after `warm` warm-ups, with `threads` OpenMP threads.
 double og_time_residual(int ni, int nj, const double* xv, const double* yv, const sg_freestream* fs,

@@ -429,11 +429,12 @@ int sg_create(int ni, int nj, const double* xv, const double* yv, const sg_frees
   const size_t naos = ctx->ncell() * ctx->nvar, nsoa = ctx->nsoa();
   const size_t npart = residual_partial_count(g) * 2 * ctx->nvar;
   if (ctx->aos_a.alloc(naos) || ctx->aos_b.alloc(naos) || ctx->q.alloc(nsoa) || ctx->r.alloc(nsoa) ||
-      ctx->v.alloc(nsoa) || ctx->w.alloc(nsoa) || ctx->partial.alloc(npart) || ctx->l2.alloc(16) || ctx->counter.alloc(1) ||
+      ctx->v.alloc(nsoa) || ctx->w.alloc(nsoa) || ctx->partial.alloc(npart) || ctx->l2.alloc(16) || ctx->counter.alloc(2) ||
       ctx->wall_cp.alloc(G.ni) || ctx->wall_cf.alloc(G.ni) || ctx->wall_f.alloc(4))
     return fail(SG_ERR_HIP);
-  // zero the SoA padding once so padded lanes never hold garbage
-  for (DevBuf* b : {&ctx->q, &ctx->r, &ctx->v, &ctx->w, &ctx->counter})
+  // zero the SoA padding once so padded lanes never hold garbage; the fused
+  // norm's two state words and its partials start at zero
+  for (DevBuf* b : {&ctx->q, &ctx->r, &ctx->v, &ctx->w, &ctx->counter, &ctx->partial})
     if (hipMemset(b->p, 0, b->n * sizeof(double)) != hipSuccess) return fail(SG_ERR_HIP);
   *out = ctx;
   return SG_OK;
@@ -545,7 +546,7 @@ int sg_residual_dev_async(sg_ctx* ctx, const double* q_dev, double* r_dev, doubl
   if (!ctx || !q_dev || !r_dev) return SG_ERR_ARG;
   // the caller's l2_dev holds nvar norms (||R_k||_2): only those are written
   SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, l2_dev ? ctx->partial.p : nullptr,
-                         reinterpret_cast<unsigned*>(ctx->counter.p), l2_dev, ctx->stream, ctx->clo, ctx->nvar),
+                         reinterpret_cast<unsigned long long*>(ctx->counter.p), l2_dev, ctx->stream, ctx->clo, ctx->nvar),
          "residual launch");
   return SG_OK;
 }
@@ -555,7 +556,7 @@ void* sg_get_stream(sg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 int sg_residual_dev(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_host) {
   if (!ctx || !q_dev || !r_dev) return SG_ERR_ARG;
   SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, ctx->partial.p,
-                         reinterpret_cast<unsigned*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
+                         reinterpret_cast<unsigned long long*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
          "residual launch");
   double l2[8];
   SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, ctx->nvar * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
@@ -570,7 +571,7 @@ int sg_residual(sg_ctx* ctx, const double* q_aos, double* r_aos, double* l2) {
   if ((rc = upload_aos(ctx, q_aos, ctx->q.p))) return rc;
   double l2h[8];
   SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, ctx->q.p, ctx->r.p, ctx->partial.p,
-                         reinterpret_cast<unsigned*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
+                         reinterpret_cast<unsigned long long*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
          "residual launch");
   SG_TRY(hipMemcpyAsync(l2h, ctx->l2.p, ctx->nvar * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
   if ((rc = download_aos(ctx, ctx->r.p, r_aos))) return rc;

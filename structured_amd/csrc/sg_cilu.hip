@@ -46,6 +46,17 @@ constexpr int Dj[NSLOT] = {0, -1, 1, 0, 0, -1, -1, 1, 1, -2, 2, 0, 0};
 constexpr int Low[6] = {11, 5, 3, 7, 9, 1};
 constexpr int Up[6] = {2, 10, 6, 4, 8, 12};
 constexpr int Back[NSLOT] = {0, 2, 1, 4, 3, 8, 7, 6, 5, 10, 9, 12, 11};
+// The 3x3-box pattern (slots 0-8; the +-2 slots dropped): four lower and
+// four upper neighbours, in the same order as above.
+constexpr int Low9[4] = {5, 3, 7, 1};
+constexpr int Up9[4] = {2, 6, 4, 8};
+template <bool NINE>
+constexpr int nn() { return NINE ? 4 : 6; }
+template <bool NINE>
+constexpr int low(int n) { return NINE ? Low9[n] : Low[n]; }
+template <bool NINE>
+constexpr int up(int n) { return NINE ? Up9[n] : Up[n]; }
+constexpr bool in_box(int s) { return s >= 0 && s < 9; }
 constexpr int slot(int di, int dj) {
   for (int s = 0; s < NSLOT; ++s)
     if (Di[s] == di && Dj[s] == dj) return s;
@@ -105,10 +116,13 @@ static size_t cilu_copy_floats(int nic, int njc, int nv, int w) {
   return (size_t)nchunk * nlev * NSLOT * nv * nv * w;
 }
 
-template <int NV, int W>
+// NINE: ILU(0) on the 3x3-box pattern (the +-2 blocks of A and their fill
+// dropped; with the first-order Jacobian of lhs_order 1 those slots hold
+// fill only), else on the 13-point stencil.
+template <int NV, int W, bool NINE>
 __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int njc, int nlev, const double* ops,
                                                                   float* rec, size_t tcopy) {
-  constexpr int B = NV * NV, RS = cilu_rs<NV, W>(), NR = 4;
+  constexpr int B = NV * NV, RS = cilu_rs<NV, W>(), NR = 4, NN = ct::nn<NINE>();
   __shared__ float ring[NR][7][W][B];  // per level: diagonal inverse, then the six upper blocks (ct::Up order)
   __shared__ double Ab[W][B];
   __shared__ double Lb[W][B];
@@ -128,8 +142,8 @@ __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int n
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) a[s] = opl[(size_t)s * B * W];
 #pragma unroll
-    for (int qi = 0; qi < 6; ++qi) {
-      const int s = ct::Low[qi], di = ct::Di[s], dj = ct::Dj[s];
+    for (int qi = 0; qi < NN; ++qi) {
+      const int s = ct::low<NINE>(qi), di = ct::Di[s], dj = ct::Dj[s];
       const int cn = c + di, jn = j + dj;
       const bool nok = valid && inside(cn, jn);
       const int ln = (L + dj + 2 * di) & (NR - 1);  // the neighbour's level in the ring
@@ -146,11 +160,11 @@ __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int n
       lds_barrier();
       if (nok) {
 #pragma unroll
-        for (int ui = 0; ui < 6; ++ui) {
-          const int ts = ct::Up[ui];
+        for (int ui = 0; ui < NN; ++ui) {
+          const int ts = ct::up<NINE>(ui);
           const int ri = di + ct::Di[ts], rj = dj + ct::Dj[ts];
           const int su = ct::slot(ri, rj);
-          if (su < 0 || !inside(c + ri, j + rj)) continue;
+          if (su < 0 || (NINE && !ct::in_box(su)) || !inside(c + ri, j + rj)) continue;
           double sum = 0.0;
 #pragma unroll
           for (int k = 0; k < NV; ++k) sum += Lb[c][r * NV + k] * (double)ring[ln][1 + ui][cs][k * NV + q];
@@ -170,13 +184,13 @@ __global__ __launch_bounds__(W * NV * NV) void cilu_factor_kernel(int nic, int n
     auto at_n = [&](int s) { return (size_t)s * SS + on; };
     auto at_t = [&](int s) { return (size_t)s * SS + ot; };
 #pragma unroll
-    for (int s = 1; s < NSLOT; ++s) {
+    for (int s = 1; s < (NINE ? 9 : NSLOT); ++s) {  // the box pattern never reads the +-2 slots
       const float v = valid ? (float)a[s] : 0.0f;
       RL[at_n(s)] = v;
       RTL[at_t(s)] = v;
     }
 #pragma unroll
-    for (int ui = 0; ui < 6; ++ui) ring[lr][1 + ui][c][e] = valid ? (float)a[ct::Up[ui]] : 0.0f;
+    for (int ui = 0; ui < NN; ++ui) ring[lr][1 + ui][c][e] = valid ? (float)a[ct::up<NINE>(ui)] : 0.0f;
     // inverse of the factored diagonal block: Gauss-Jordan with partial
     // pivoting on [D | I], column-parallel (thread e < 2 NV of a cell owns
     // column e; the pivot column is broadcast through LDS each step)
@@ -253,7 +267,7 @@ __device__ __forceinline__ void load_row(const float* p, double* out) {
 // outside the chunk or the grid and for invalid cells (the transposed pass
 // reads the cell's own slot, zero for an absent neighbour), so nothing is
 // masked and all the loads of a level are in flight together.
-template <int NV, int W, bool TR, bool ASC, bool DIAG>
+template <int NV, int W, bool NINE, bool TR, bool ASC, bool DIAG>
 __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
                                            int k, int c, int L, const double* src, double* f, double* d, double& x) {
   constexpr int RS = cilu_rs<NV, W>();
@@ -263,8 +277,8 @@ __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, in
   const float* RL = R + (size_t)(lok ? L : 0) * RS;
   auto row = [&](const float* Rlev, int s, int r, int col) { return Rlev + ((size_t)(s * NV + r) * W + col) * NV; };
 #pragma unroll
-  for (int n = 0; n < 6; ++n) {
-    const int s = ASC ? ct::Low[n] : ct::Up[n];
+  for (int n = 0; n < ct::nn<NINE>(); ++n) {
+    const int s = ASC ? ct::low<NINE>(n) : ct::up<NINE>(n);
     if (!TR) {
       load_row<NV, W>(row(RL, s, k, c), f + n * NV);
     } else {
@@ -281,7 +295,7 @@ __device__ __forceinline__ void cilu_fetch(const float* R, int nlev, int njc, in
 
 // One level of a triangular pass with its operands already in registers.
 // Returns the thread's solved value (stored to z by the caller).
-template <int NV, int W, bool ASC, bool DIAG>
+template <int NV, int W, bool NINE, bool ASC, bool DIAG>
 __device__ __forceinline__ double cilu_level(int L, int njc, int wk, int k, int c, const double* f, const double* d,
                                              double x, double (*ring)[NV][W], double (*tmp)[W]) {
   constexpr int NR = 8;
@@ -289,8 +303,8 @@ __device__ __forceinline__ double cilu_level(int L, int njc, int wk, int k, int 
   const bool valid = c < wk && j >= 0 && j < njc;
   double a0 = x, a1 = 0.0;
 #pragma unroll
-  for (int n = 0; n < 6; ++n) {
-    const int s = ASC ? ct::Low[n] : ct::Up[n];
+  for (int n = 0; n < ct::nn<NINE>(); ++n) {
+    const int s = ASC ? ct::low<NINE>(n) : ct::up<NINE>(n);
     const int cn = c + ct::Di[s], jn = j + ct::Dj[s];
     // out-of-chunk / out-of-grid neighbours carry f = 0: any in-range ring
     // entry (all finite) will do
@@ -349,25 +363,25 @@ static_assert(ZB % NBUF == 0 && PROG_EVERY % ZB == 0, "batch sizes");
 // extra levels hold no valid cell.  Thread 0 reports the steps done to
 // *prog (tag | steps) for the helpers, right after a level's loads, so that
 // the report's store is not waited for before the next level.
-template <int NV, int W, bool TR, bool ASC, bool DIAG>
+template <int NV, int W, bool NINE, bool TR, bool ASC, bool DIAG>
 __device__ __forceinline__ void cilu_pass(const float* R, int nlev, int njc, int ldi, size_t plane, int i0, int wk,
                                           int k, int c, const double* src, double* z, double (*ring)[NV][W],
                                           double (*tmp)[W], unsigned* prog, unsigned tag) {
-  double f[NBUF][6 * NV], d[NBUF][NV], x[NBUF];
+  double f[NBUF][ct::nn<NINE>() * NV], d[NBUF][NV], x[NBUF];
   auto lev = [&](int step) { return ASC ? step : nlev - 1 - step; };
 #pragma unroll
   for (int u = 0; u < NBUF - 1; ++u)
-    cilu_fetch<NV, W, TR, ASC, DIAG>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(u), src, f[u], d[u], x[u]);
+    cilu_fetch<NV, W, NINE, TR, ASC, DIAG>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(u), src, f[u], d[u], x[u]);
   for (int step = 0; step < nlev; step += ZB) {
     double zv[ZB];
 #pragma unroll
     for (int u = 0; u < ZB; ++u) {
       const int cur = u % NBUF, nb = (u + NBUF - 1) % NBUF;
-      cilu_fetch<NV, W, TR, ASC, DIAG>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + u + NBUF - 1), src, f[nb],
-                                       d[nb], x[nb]);
+      cilu_fetch<NV, W, NINE, TR, ASC, DIAG>(R, nlev, njc, ldi, plane, i0, wk, k, c, lev(step + u + NBUF - 1), src,
+                                             f[nb], d[nb], x[nb]);
       if (u == 0 && prog && threadIdx.x == 0 && (step % PROG_EVERY) == 0)
         __hip_atomic_store(prog, tag + (unsigned)step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      zv[u] = cilu_level<NV, W, ASC, DIAG>(lev(step + u), njc, wk, k, c, f[cur], d[cur], x[cur], ring, tmp);
+      zv[u] = cilu_level<NV, W, NINE, ASC, DIAG>(lev(step + u), njc, wk, k, c, f[cur], d[cur], x[cur], ring, tmp);
     }
 #pragma unroll
     for (int u = 0; u < ZB; ++u) {
@@ -381,21 +395,21 @@ __device__ __forceinline__ void cilu_pass(const float* R, int nlev, int njc, int
 // pass reads, level by level in the pass's order, steps h, h + NHELP, ...
 // Pass p (0 ascending, 1 descending) of M^-1 reads the lower slots, then the
 // upper slots and the diagonal; M^-T reads them the other way round.
-template <int NV, int W, bool TR>
+template <int NV, int W, bool NINE, bool TR>
 __device__ void cilu_helper(const float* R, int nlev, const unsigned* prog, unsigned epoch, int h, unsigned* sink) {
-  constexpr int NT = NV * W, SLOTF = NV * NV * W, RS = cilu_rs<NV, W>();
-  constexpr int LPS = (SLOTF + 31) / 32;          // 128-byte lines per slot
-  constexpr int LPT = (7 * LPS + NT - 1) / NT;    // lines per thread and level
+  constexpr int NT = NV * W, SLOTF = NV * NV * W, RS = cilu_rs<NV, W>(), NN = ct::nn<NINE>();
+  constexpr int LPS = (SLOTF + 31) / 32;               // 128-byte lines per slot
+  constexpr int LPT = ((NN + 1) * LPS + NT - 1) / NT;  // lines per thread and level
   float acc = 0.f;
   int budget = HSPIN_TOTAL;
   for (int pass = 0; pass < 2; ++pass) {
     const bool upper = (pass == 0) == TR;  // upper slots and the diagonal
-    const int nslot = upper ? 7 : 6;
+    const int nslot = upper ? NN + 1 : NN;
     int off[LPT];
 #pragma unroll
     for (int u = 0; u < LPT; ++u) {
       const int t = threadIdx.x + u * NT, si = t / LPS, ln = t % LPS;
-      const int s = si >= nslot ? -1 : upper ? (si == 6 ? 0 : ct::Up[si]) : ct::Low[si];
+      const int s = si >= nslot ? -1 : upper ? (si == NN ? 0 : ct::up<NINE>(si)) : ct::low<NINE>(si);
       off[u] = (s < 0 || ln * 32 >= SLOTF) ? -1 : s * SLOTF + ln * 32;
     }
     const unsigned base = (unsigned)(pass * nlev);
@@ -430,7 +444,7 @@ __device__ void cilu_helper(const float* R, int nlev, const unsigned* prog, unsi
 // Work-groups [0, nchunk) solve their chunk; with helpers (prog non-null),
 // work-group npad (1 + m) + chunk is helper m of the chunk, npad being the
 // chunk count rounded up to a multiple of 8, so it runs on the chunk's XCD.
-template <int NV, int W, bool TR>
+template <int NV, int W, bool NINE, bool TR>
 __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, int ldi, int nlev, const float* rec,
                                                             size_t tcopy, const double* b, double* z, unsigned* prog,
                                                             unsigned epoch) {
@@ -444,7 +458,7 @@ __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, in
   if (chunk >= nchunk) return;
   const float* R = rec + (TR ? tcopy : 0) + (size_t)chunk * nlev * RS;
   if (role > 0) {
-    cilu_helper<NV, W, TR>(R, nlev, prog + chunk, epoch, role - 1, prog + npad);
+    cilu_helper<NV, W, NINE, TR>(R, nlev, prog + chunk, epoch, role - 1, prog + npad);
     return;
   }
   const int i0 = chunk * W, wk = min(W, nic - i0);
@@ -457,11 +471,11 @@ __global__ __launch_bounds__(NV * W) void cilu_apply_kernel(int nic, int njc, in
   __syncthreads();
   // non-transposed: unit-lower L ascending, then D + U descending;
   // transposed: (D + U)^T ascending, then unit-lower L^T descending
-  cilu_pass<NV, W, TR, true, TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, b, z, ring, tmp, pc, tag);
+  cilu_pass<NV, W, NINE, TR, true, TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, b, z, ring, tmp, pc, tag);
   // the second pass reads what the first stored: complete every store first
   __syncthreads();
-  cilu_pass<NV, W, TR, false, !TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, z, z, ring, tmp, pc,
-                                   tag + (unsigned)nlev);
+  cilu_pass<NV, W, NINE, TR, false, !TR>(R, nlev, njc, ldi, plane, i0, wk, k, c, z, z, ring, tmp, pc,
+                                         tag + (unsigned)nlev);
 }
 
 // progress words of the helpers: one per chunk (padded to a multiple of 8)
@@ -480,7 +494,7 @@ size_t cilu_size(int nic, int njc, int nv, int w) {
   return cilu_ops_offset(nic, njc, nv, w) + cilu_copy_floats(nic, njc, nv, w);
 }
 
-template <int NV, int W>
+template <int NV, int W, bool NINE>
 static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const double* blocks, double* store,
                           hipStream_t s) {
   const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2;
@@ -489,14 +503,14 @@ static void cilu_factor_t(int nic, int njc, int ldi, const double* idt, const do
   const size_t nblk = std::min<size_t>((total + 255) / 256, (size_t)1 << 20);
   hipLaunchKernelGGL((cilu_gather_kernel<NV, W>), dim3((unsigned)nblk), dim3(256), 0, s, nic, njc, ldi, nlev, idt,
                      blocks, ops, total);
-  hipLaunchKernelGGL((cilu_factor_kernel<NV, W>), dim3(nchunk), dim3(W * NV * NV), 0, s, nic, njc, nlev, ops,
+  hipLaunchKernelGGL((cilu_factor_kernel<NV, W, NINE>), dim3(nchunk), dim3(W * NV * NV), 0, s, nic, njc, nlev, ops,
                      reinterpret_cast<float*>(store), total);
 }
 
 // Launch tag of an apply: helpers only trust progress words that carry it.
 static std::atomic<unsigned> g_apply_epoch{0};
 
-template <int NV, int W>
+template <int NV, int W, bool NINE>
 static void cilu_apply_t(int nic, int njc, int ldi, double* store, const double* b, double* z, bool tr,
                          hipStream_t s) {
   const int nchunk = (nic + W - 1) / W, nlev = njc + 2 * W - 2, npad = (nchunk + 7) & ~7;
@@ -510,40 +524,54 @@ static void cilu_apply_t(int nic, int njc, int ldi, double* store, const double*
   const dim3 grid(helpers ? npad * (1 + NHELP) : nchunk);
   if (!helpers) prog = nullptr, epoch = 0;
   if (tr)
-    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, true>), grid, dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec, tcopy, b,
-                       z, prog, epoch);
+    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, NINE, true>), grid, dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec,
+                       tcopy, b, z, prog, epoch);
   else
-    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, false>), grid, dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec, tcopy,
-                       b, z, prog, epoch);
+    hipLaunchKernelGGL((cilu_apply_kernel<NV, W, NINE, false>), grid, dim3(NV * W), 0, s, nic, njc, ldi, nlev, rec,
+                       tcopy, b, z, prog, epoch);
 }
 
 bool cilu_width_ok(int w) { return w == 8 || w == 16 || w == 32; }
 
-hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, const double* idt, const double* blocks, double* store,
-                       hipStream_t s) {
+template <bool NINE>
+static void cilu_factor_n(int nic, int njc, int ldi, int nv, int w, const double* idt, const double* blocks,
+                          double* store, hipStream_t s) {
   if (nv == 5) {
-    if (w == 8) cilu_factor_t<5, 8>(nic, njc, ldi, idt, blocks, store, s);
-    else if (w == 16) cilu_factor_t<5, 16>(nic, njc, ldi, idt, blocks, store, s);
-    else cilu_factor_t<5, 32>(nic, njc, ldi, idt, blocks, store, s);
+    if (w == 8) cilu_factor_t<5, 8, NINE>(nic, njc, ldi, idt, blocks, store, s);
+    else if (w == 16) cilu_factor_t<5, 16, NINE>(nic, njc, ldi, idt, blocks, store, s);
+    else cilu_factor_t<5, 32, NINE>(nic, njc, ldi, idt, blocks, store, s);
   } else {
-    if (w == 8) cilu_factor_t<4, 8>(nic, njc, ldi, idt, blocks, store, s);
-    else if (w == 16) cilu_factor_t<4, 16>(nic, njc, ldi, idt, blocks, store, s);
-    else cilu_factor_t<4, 32>(nic, njc, ldi, idt, blocks, store, s);
+    if (w == 8) cilu_factor_t<4, 8, NINE>(nic, njc, ldi, idt, blocks, store, s);
+    else if (w == 16) cilu_factor_t<4, 16, NINE>(nic, njc, ldi, idt, blocks, store, s);
+    else cilu_factor_t<4, 32, NINE>(nic, njc, ldi, idt, blocks, store, s);
   }
+}
+
+template <bool NINE>
+static void cilu_apply_n(int nic, int njc, int ldi, int nv, int w, double* store, const double* b, double* z,
+                         bool transpose, hipStream_t s) {
+  if (nv == 5) {
+    if (w == 8) cilu_apply_t<5, 8, NINE>(nic, njc, ldi, store, b, z, transpose, s);
+    else if (w == 16) cilu_apply_t<5, 16, NINE>(nic, njc, ldi, store, b, z, transpose, s);
+    else cilu_apply_t<5, 32, NINE>(nic, njc, ldi, store, b, z, transpose, s);
+  } else {
+    if (w == 8) cilu_apply_t<4, 8, NINE>(nic, njc, ldi, store, b, z, transpose, s);
+    else if (w == 16) cilu_apply_t<4, 16, NINE>(nic, njc, ldi, store, b, z, transpose, s);
+    else cilu_apply_t<4, 32, NINE>(nic, njc, ldi, store, b, z, transpose, s);
+  }
+}
+
+hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, bool nine, const double* idt, const double* blocks,
+                       double* store, hipStream_t s) {
+  if (nine) cilu_factor_n<true>(nic, njc, ldi, nv, w, idt, blocks, store, s);
+  else cilu_factor_n<false>(nic, njc, ldi, nv, w, idt, blocks, store, s);
   return hipGetLastError();
 }
 
-hipError_t cilu_apply(int nic, int njc, int ldi, int nv, int w, double* store, const double* b, double* z,
+hipError_t cilu_apply(int nic, int njc, int ldi, int nv, int w, bool nine, double* store, const double* b, double* z,
                       bool transpose, hipStream_t s) {
-  if (nv == 5) {
-    if (w == 8) cilu_apply_t<5, 8>(nic, njc, ldi, store, b, z, transpose, s);
-    else if (w == 16) cilu_apply_t<5, 16>(nic, njc, ldi, store, b, z, transpose, s);
-    else cilu_apply_t<5, 32>(nic, njc, ldi, store, b, z, transpose, s);
-  } else {
-    if (w == 8) cilu_apply_t<4, 8>(nic, njc, ldi, store, b, z, transpose, s);
-    else if (w == 16) cilu_apply_t<4, 16>(nic, njc, ldi, store, b, z, transpose, s);
-    else cilu_apply_t<4, 32>(nic, njc, ldi, store, b, z, transpose, s);
-  }
+  if (nine) cilu_apply_n<true>(nic, njc, ldi, nv, w, store, b, z, transpose, s);
+  else cilu_apply_n<false>(nic, njc, ldi, nv, w, store, b, z, transpose, s);
   return hipGetLastError();
 }
 

@@ -69,7 +69,7 @@ int ensure_buffers(sg_ctx* ctx) {
 
 // Enqueues one step; the norms of the last stage's residual land in ctx->l2.
 int enqueue_step(sg_ctx* ctx, double* q, int scheme, double cfl) {
-  unsigned* counter = reinterpret_cast<unsigned*>(ctx->counter.p);
+  unsigned long long* counter = reinterpret_cast<unsigned long long*>(ctx->counter.p);
   double* R = ctx->ex_r.p;
   auto residual = [&](const double* in) {
     return launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, in, R, ctx->partial.p, counter, ctx->l2.p, ctx->stream,

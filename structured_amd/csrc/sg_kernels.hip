@@ -38,20 +38,20 @@ static void launch_cl(dim3 grid, hipStream_t s, const DevGeom& g, const DevPhys&
 }
 
 hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
-                           double* partial, unsigned* counter, double* l2, hipStream_t s, const DevClosure& clo,
+                           double* partial, unsigned long long* counter, double* l2, hipStream_t s, const DevClosure& clo,
                            int l2_count) {
   dim3 grid = tile_grid(g);
   Field<double> fq{q};
   OutField<double> fo{r};
   const int nv2 = 2 * (sa ? 5 : 4);
   if (l2_count <= 0 || l2_count > nv2) l2_count = nv2;
-  // with a counter the norms are finalised inside the residual kernel
-  const NormOut no{counter, (partial && counter) ? l2 : nullptr, l2_count};
+  // the norms are finalised inside the residual kernel; partials without a
+  // state pair and a norm buffer are not a supported combination
+  if ((l2 != nullptr) != (partial != nullptr && counter != nullptr) || (partial && !l2)) return hipErrorInvalidValue;
+  const NormOut no{counter, l2, l2_count};
   if (sa) launch_cl<double, true>(grid, s, g, ph, bc, fq, fo, partial, nullptr, clo, no);
   else launch_cl<double, false>(grid, s, g, ph, bc, fq, fo, partial, nullptr, DevClosure(), no);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || !partial || !l2 || counter) return e;
-  return launch_norm_finalize(partial, (int)residual_partial_count(g), nv2, l2_count, l2, s);
+  return hipGetLastError();
 }
 
 hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,
@@ -61,38 +61,6 @@ hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool 
   OutField<Dual> fo{r, jv};
   if (sa) launch_cl<Dual, true>(grid, s, g, ph, bc, fq, fo, nullptr, beta_dot, clo);
   else launch_cl<Dual, false>(grid, s, g, ph, bc, fq, fo, nullptr, nullptr, DevClosure());
-  return hipGetLastError();
-}
-
-// Sum of the per-block partials, stored component-major (partial[k * nblocks
-// + b]): thread t adds blocks t, t + 1024, ... of every component with
-// coalesced loads, then a fixed butterfly per wavefront and a fixed order
-// over the 16 wavefronts, so the result is bitwise reproducible; then sqrt.
-__global__ __launch_bounds__(1024) void norm_finalize_kernel(const double* partial, int nblocks, int nv, int nout,
-                                                             double* l2) {
-  __shared__ double red[16][10];
-  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int b = threadIdx.x; b < nblocks; b += 1024)
-#pragma unroll
-    for (int k = 0; k < 10; ++k)
-      if (k < nv) acc[k] += partial[(size_t)k * nblocks + b];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < 10; ++k) {
-    double s = acc[k];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) red[wv][k] = s;
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < nv && (int)threadIdx.x < nout) {
-    double s = 0.0;
-    for (int w = 0; w < 16; ++w) s += red[w][threadIdx.x];
-    l2[threadIdx.x] = sqrt(s);
-  }
-}
-
-hipError_t launch_norm_finalize(const double* partial, int nblocks, int nv, int nout, double* l2, hipStream_t s) {
-  hipLaunchKernelGGL(norm_finalize_kernel, dim3(1), dim3(1024), 0, s, partial, nblocks, nv, nout, l2);
   return hipGetLastError();
 }
 

@@ -7,19 +7,19 @@
 namespace sg {
 
 size_t residual_partial_count(const DevGeom& g);
-// partial: per-block norm partials (2 nv per block, or null); with l2
-// (device: the L2 norms of R, then of V R) the norms are finalised by the
-// residual kernel's last block when counter (a device unsigned kept at 0
-// between launches) is given, else by a second kernel.  l2_count entries of
-// l2 are written (0: all 2 nv; nv: the norms of R only).
+// Norms (optional, all three of partial, counter and l2 or none): partial
+// holds 2 nv per-block partial sums, counter two device words (start-ticket
+// count and launch parity); counter must be 0 and partial +0.0 when they are
+// first used, and then stay with the grid they were used for.  The residual
+// kernel's block that starts last finalises the L2 norms of R, then of V R,
+// into l2 (device): l2_count entries (0: all 2 nv; nv: the norms of R only).
 hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
-                           double* partial, unsigned* counter, double* l2, hipStream_t s,
+                           double* partial, unsigned long long* counter, double* l2, hipStream_t s,
                            const DevClosure& clo = DevClosure(), int l2_count = 0);
 // J.v (+ (dR/dbeta).beta_dot when beta_dot is not null; v may be all zero)
 hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,
                      double* r, double* jv, hipStream_t s, const double* beta_dot = nullptr,
                      const DevClosure& clo = DevClosure());
-hipError_t launch_norm_finalize(const double* partial, int nblocks, int nv, int nout, double* l2, hipStream_t s);
 hipError_t launch_dt(const DevGeom& g, double mu_inf, int nv, const double* q, const double* dsmin, double cfl,
                      double* dt, hipStream_t s);
 hipError_t launch_aos_to_soa(int nic, int njc, int ldi, int nv, const double* aos, double* soa, hipStream_t s);

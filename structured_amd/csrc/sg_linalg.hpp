@@ -40,16 +40,17 @@ hipError_t lilu_factor(int nic, int njc, int ldi, int nv, const double* idt, con
                        hipStream_t s);
 hipError_t lilu_apply(int nic, int njc, int ldi, int nv, const double* ilu, const double* b, double* z, bool transpose,
                       hipStream_t s);
-// chunked line block ILU(0) (sg_cilu.hip): chunks of w in {8, 16, 32} columns;
-// cilu_size is the storage in doubles that cilu_factor writes and cilu_apply
-// reads: two FP32 copies of the factors (plain and transposed), the apply's
-// progress words (written by every apply), and the factorisation's operands
-// gathered in level order
+// chunked line block ILU(0) (sg_cilu.hip): chunks of w in {8, 16, 32} columns,
+// on the 13-point stencil or (nine) the 3x3-box pattern; cilu_size is the
+// storage in doubles that cilu_factor writes and cilu_apply reads: two FP32
+// copies of the factors (plain and transposed), the apply's progress words
+// (written by every apply), and the factorisation's operands gathered in
+// level order
 bool cilu_width_ok(int w);
 size_t cilu_size(int nic, int njc, int nv, int w);
-hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, const double* idt, const double* blocks, double* rec,
-                       hipStream_t s);
-hipError_t cilu_apply(int nic, int njc, int ldi, int nv, int w, double* store, const double* b, double* z,
+hipError_t cilu_factor(int nic, int njc, int ldi, int nv, int w, bool nine, const double* idt, const double* blocks,
+                       double* rec, hipStream_t s);
+hipError_t cilu_apply(int nic, int njc, int ldi, int nv, int w, bool nine, double* store, const double* b, double* z,
                       bool transpose, hipStream_t s);
 // adjoint helpers
 hipError_t jt_gather(int nic, int njc, int ldi, int nv, const double* blocks, const double* psi, double* out,

@@ -350,12 +350,18 @@ __device__ __forceinline__ SaTerms<S> sa_terms(S r, S n, S ml, const S* gu, cons
   return t;
 }
 
-// Fused norm output: with l2 set, the last block finalises the norms.
+// Fused norm output: with l2 set, the block that starts last finalises the
+// norms.  state[0] counts the blocks that have started the launch (0 between
+// launches); state[1] is the launch parity, flipped by every fused launch.
 struct NormOut {
-  unsigned* counter;  // arrival count, 0 between launches
+  unsigned long long* state;
   double* l2;         // ||R_k||_2 (k < NV), then ||V R_k||_2 (NV <= k < 2 NV)
   int count;          // entries of l2 written: NV (caller's buffer) or 2 NV
 };
+
+// Spin budget of the finalising block per thread (polls beyond the first
+// load of each partial); past it the norms are written as NaN.
+constexpr int kNormSpinLimit = 1 << 22;
 
 // CL: closure kind of beta (ClosureKind); CLOSURE_MLP stages the hidden
 // layer of the network (and its tangent) in the same LDS area.
@@ -387,6 +393,9 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
   const int i0 = (tile % gridDim.x) * OX, j0 = (tile / gridDim.x) * OY;
   const int nic = g.nic, njc = g.njc;
   const double nan = __builtin_nan("");
+  const bool fused = partial && nout.l2;
+  // [0]: this block's start ticket, [1]: the launch parity (fused norm only)
+  __shared__ unsigned long long ticket[2];
 
   // ---- stage primitives (+ transport properties) of tile + halo in LDS.
   // A thread stages at most NPASS positions; the conservative values of its
@@ -407,6 +416,13 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
 #pragma unroll
       for (int k = 0; k < NQ; ++k) qraw[pass][k] = q[k * g.plane + idx];
     }
+  }
+  // Fused norm: take a start ticket while the tile's loads are in flight.
+  // The block holding the last ticket finalises the norms: every other block
+  // has started by then, so its wait for their partials always ends.
+  if (fused && tid == 0) {
+    ticket[0] = __hip_atomic_fetch_add(nout.state, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ticket[1] = __hip_atomic_load(nout.state + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 #pragma unroll
   for (int pass = 0; pass < NPASS; ++pass) {
@@ -664,16 +680,10 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
       sq[NV + k] = sq[k] * V * V;
     }
   }
-  // R is stored after the block's norm partials and its arrival are
-  // acknowledged: the wait for those then does not include the block's R
-  // stores, which drain while the last block finalises.
-  auto store_residual = [&]() {
-    if (owns) {
+  if (owns) {
 #pragma unroll
-      for (int k = 0; k < NV; ++k) out.put(k * g.plane + oidx, res[k]);
-    }
-  };
-  if (!(partial && nout.l2)) store_residual();
+    for (int k = 0; k < NV; ++k) out.put(k * g.plane + oidx, res[k]);
+  }
   // ---- block partial sums of R^2 (fixed order: wavefront DPP sum, then waves)
   if (partial) {
     const int lane = tid & 63, wv = tid >> 6;
@@ -685,60 +695,71 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
       for (int k = 0; k < 2 * NV; ++k) red[wv][k] = sq[k];
     }
     __syncthreads();
+    // Fused: the sign bit of each stored partial (a sum of squares, >= 0)
+    // carries the launch parity -- set on launches of parity 0 -- so the
+    // finalising block recognises this launch's value in every slot from
+    // that slot alone.  No ordering between slots, or between a slot and the
+    // ticket, is needed: each slot is one 64-bit atomic store, read by
+    // atomic loads of the same location, and launches on a stream are
+    // ordered by the kernel boundary.  Slots start at +0.0 (parity 1), so a
+    // slot never written in this context never passes for parity 0.
+    const unsigned long long par = fused ? (ticket[1] & 1ull) : 1ull;
     if (tid < 2 * NV) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
-      // device-coherent store, acknowledged before the arrival count below
+      if (par == 0) s = -s;
       __hip_atomic_store(&partial[(size_t)tid * nb + bid], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (nout.l2) {
-      // The last block to arrive sums the partials of all blocks in a fixed
-      // order (so the norm does not depend on which block is last) and resets
-      // the arrival counter for the next launch.  Partials and counter are
-      // device-coherent atomics; each block's stores complete before its
-      // arrival, and the last block reads them after seeing every arrival.
-      __shared__ unsigned last;
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      if (tid == 0)
-        last = __hip_atomic_fetch_add(nout.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (unsigned)(nb - 1);
-      __syncthreads();
-      store_residual();
-      if (last) {
-        double acc[2 * NV];
+    if (fused && ticket[0] == (unsigned long long)(nb - 1)) {
+      // The block that started last sums the partials of all blocks in a
+      // fixed order (so the norm does not depend on which block it is),
+      // polling each slot until it holds this launch's value.
+      const bool want_neg = par == 0;
+      int spins = 0;
+      double acc[2 * NV];
 #pragma unroll
-        for (int k = 0; k < 2 * NV; ++k) acc[k] = 0.0;
-        // four rows of blocks per round, all loads of a round in flight together
-        for (int b0 = 0; b0 < nb; b0 += 4 * NT) {
-          double v[4][2 * NV];
+      for (int k = 0; k < 2 * NV; ++k) acc[k] = 0.0;
+      // two rows of blocks per round, all loads of a round in flight together
+      for (int b0 = 0; b0 < nb; b0 += 2 * NT) {
+        double v[2][2 * NV];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int b = b0 + u * NT + tid;
+        for (int u = 0; u < 2; ++u) {
+          const int b = b0 + u * NT + tid;
 #pragma unroll
-            for (int k = 0; k < 2 * NV; ++k)
-              v[u][k] = b < nb ? __hip_atomic_load(&partial[(size_t)k * nb + b], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT)
-                               : 0.0;
+          for (int k = 0; k < 2 * NV; ++k)
+            v[u][k] = b < nb ? __hip_atomic_load(&partial[(size_t)k * nb + b], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                             : (want_neg ? -0.0 : 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int b = b0 + u * NT + tid;
+#pragma unroll
+          for (int k = 0; k < 2 * NV; ++k) {
+            while (__builtin_signbit(v[u][k]) != want_neg) {
+              if (++spins > kNormSpinLimit) { v[u][k] = nan; break; }
+              v[u][k] = __hip_atomic_load(&partial[(size_t)k * nb + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            acc[k] += fabs(v[u][k]);
           }
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int k = 0; k < 2 * NV; ++k) acc[k] += v[u][k];
         }
-        __syncthreads();
+      }
 #pragma unroll
-        for (int k = 0; k < 2 * NV; ++k) acc[k] = wave_sum_lane63(acc[k]);
-        if (lane == 63) {
+      for (int k = 0; k < 2 * NV; ++k) acc[k] = wave_sum_lane63(acc[k]);
+      __syncthreads();  // red[] is reused below; every wave of this block reached here
+      if (lane == 63) {
 #pragma unroll
-          for (int k = 0; k < 2 * NV; ++k) red[wv][k] = acc[k];
-        }
-        __syncthreads();
-        if (tid < 2 * NV && tid < nout.count) {
-          double s = 0.0;
-          for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
-          nout.l2[tid] = sqrt(s);
-        }
-        if (tid == 0) __hip_atomic_store(nout.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < 2 * NV; ++k) red[wv][k] = acc[k];
+      }
+      __syncthreads();
+      if (tid < 2 * NV && tid < nout.count) {
+        double s = 0.0;
+        for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
+        nout.l2[tid] = sqrt(s);
+      }
+      if (tid == 0) {  // ready for the next launch: no ticket taken, parity flipped
+        __hip_atomic_store(nout.state, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(nout.state + 1, par ^ 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   }

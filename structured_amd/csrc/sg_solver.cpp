@@ -52,7 +52,8 @@ struct SolverState {
   bool have_lhs = false, have_rhs = false;
   int fact_precond = -1, fact_width = 0;  // settings the current factors were built with
   int precond = 5;           // 0 none, 1 block Jacobi, 2 j-line, 3 line Gauss-Seidel, 4 multicolour ILU(0),
-                             // 5 level-scheduled natural-order ILU(0), 6 chunked line ILU(0)
+                             // 5 level-scheduled natural-order ILU(0), 6 chunked line ILU(0) on the
+                             // 13-point stencil, 7 the same on the 3x3-box pattern
   int sweeps = 1;
   double pc_boost = 1.0;     // preconditioner uses idt * pc_boost
   int lhs_order = 0;         // > 0: preconditioner blocks from the residual at this MUSCL order
@@ -113,8 +114,9 @@ int precond_apply(sg_ctx* ctx, const double* v, double* z) {
     SG_TRY(lilu_apply(nic, njc, ldi, nv, S->ilu.p, v, z, S->transpose, ctx->stream), "ilu apply");
     return SG_OK;
   }
-  if (S->precond == 6) {
-    SG_TRY(cilu_apply(nic, njc, ldi, nv, S->pc_width, S->cilu.p, v, z, S->transpose, ctx->stream), "cilu apply");
+  if (S->precond == 6 || S->precond == 7) {
+    SG_TRY(cilu_apply(nic, njc, ldi, nv, S->pc_width, S->precond == 7, S->cilu.p, v, z, S->transpose, ctx->stream),
+           "cilu apply");
     return SG_OK;
   }
   if (S->precond <= 2 && !S->transpose) {
@@ -126,7 +128,7 @@ int precond_apply(sg_ctx* ctx, const double* v, double* z) {
     else SG_TRY(ilu_apply(nic, njc, ldi, nv, S->ilu.p, v, z, ctx->stream), "ilu apply");
     return SG_OK;
   }
-  if (S->transpose) return sg_fail(ctx, SG_ERR_ARG, "transposed solves support precond 0, 4 and 5 only");
+  if (S->transpose) return sg_fail(ctx, SG_ERR_ARG, "transposed solves support precond 0, 4, 5, 6 and 7 only");
   if (S->precond == 5) {
     SG_TRY(lilu_apply(nic, njc, ldi, nv, S->ilu.p, v, z, S->transpose, ctx->stream), "ilu apply");
     return SG_OK;
@@ -197,10 +199,11 @@ int factor_preconditioner(sg_ctx* ctx, const double* idt_pc) {
     SG_TRY(lilu_factor(nic, njc, ldi, nv, pc_idt, blk, S->ilu.p, ctx->stream), "ilu factor");
     return SG_OK;
   }
-  if (S->precond == 6) {
+  if (S->precond == 6 || S->precond == 7) {
     const size_t need = cilu_size(nic, njc, nv, S->pc_width);
     if (S->cilu.n < need && S->cilu.alloc(need) != hipSuccess) return sg_fail(ctx, SG_ERR_HIP, "allocation failed");
-    SG_TRY(cilu_factor(nic, njc, ldi, nv, S->pc_width, pc_idt, blk, S->cilu.p, ctx->stream), "cilu factor");
+    SG_TRY(cilu_factor(nic, njc, ldi, nv, S->pc_width, S->precond == 7, pc_idt, blk, S->cilu.p, ctx->stream),
+           "cilu factor");
     return SG_OK;
   }
   SG_TRY(line_factor(nic, njc, ldi, nv, pc_idt, blk, S->fac.p, S->precond == 1, ctx->stream), "factor");
@@ -343,7 +346,7 @@ extern "C" {
 
 // ---- LinearSolverGMRES: the reference's linear-solver interface ----------
 int sg_ls_preallocate(sg_ctx* ctx, int restart, int precond, int maxit, double rtol) {
-  if (!ctx || restart < 1 || restart > 200 || precond < 0 || precond > 6) return SG_ERR_ARG;
+  if (!ctx || restart < 1 || restart > 200 || precond < 0 || precond > 7) return SG_ERR_ARG;
   int rc = ensure(ctx, restart);
   if (rc) return rc;
   ctx->solver->have_lhs = false;
@@ -446,7 +449,7 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     SG_TRY(launch_dt(ctx->g, ctx->ph.mu_inf, nv, q, ctx->dsmin.p, cfl, S->dtv.p, ctx->stream), "dt");
     SG_TRY(invert_dt(S->dtv.p, S->idt.p, ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->stream), "invert");
     SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q, S->rhs.p, ctx->partial.p,
-                           reinterpret_cast<unsigned*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
+                           reinterpret_cast<unsigned long long*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
            "residual");
     SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
     SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
@@ -563,7 +566,7 @@ int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos, int tra
     return sg_fail(ctx, SG_ERR_STATE, "set_lhs (after the last preallocate / set_pc / solve / adjoint call) must "
                                       "precede precond_apply");
   SolverState* S = ctx->solver;
-  if (S->fact_precond != S->precond || (S->precond == 6 && (S->fact_width != S->pc_width || !S->cilu.p)))
+  if (S->fact_precond != S->precond || (S->precond >= 6 && (S->fact_width != S->pc_width || !S->cilu.p)))
     return sg_fail(ctx, SG_ERR_STATE, "preconditioner factors do not match the current settings");
   int rc;
   if ((rc = upload(ctx, v_aos, S->tmp.p))) return rc;
@@ -648,8 +651,8 @@ int sg_jtv(sg_ctx* ctx, const double* q_aos, const double* psi_aos, double* jtps
 int sg_adjoint_solve(sg_ctx* ctx, const double* q_aos, const double* rhs_aos, double* psi_aos,
                      const sg_adjoint_opts* o, sg_adjoint_report* rep) {
   if (!ctx || !q_aos || !rhs_aos || !psi_aos || !o) return SG_ERR_ARG;
-  if (o->precond != 0 && o->precond != 4 && o->precond != 5 && o->precond != 6)
-    return sg_fail(ctx, SG_ERR_ARG, "adjoint precond must be 0, 4, 5 or 6");
+  if (o->precond != 0 && o->precond != 4 && o->precond != 5 && o->precond != 6 && o->precond != 7)
+    return sg_fail(ctx, SG_ERR_ARG, "adjoint precond must be 0, 4, 5, 6 or 7");
   if (o->lhs_order < 0 || o->lhs_order > 2) return sg_fail(ctx, SG_ERR_ARG, "adjoint lhs_order must be 0, 1 or 2");
   int rc;
   if ((rc = sg_ls_preallocate(ctx, o->restart, o->precond, o->maxit, o->rtol))) return rc;

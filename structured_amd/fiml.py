@@ -160,7 +160,7 @@ def stage_invert(args):
     for k in range(lo, min(hi, len(bumps))):
         h, w = bumps[k]
         case = bump_case(ni, nj, h, w, re=3e6)
-        r = run_case(case, coef, invert_iterations=args.iterations, verbose=True)
+        r = run_case(case, coef, invert_iterations=args.iterations, verbose=True, resolve=dict(RESOLVE, verbose=1))
         half = case.njc // 2
         np.savez_compressed(os.path.join(args.out, "case_%02d.npz" % k),
                             features=r["features"][:, :half].astype(np.float32),
@@ -213,10 +213,82 @@ def stage_test(args):
             json.dump(out, f)
 
 
+def stage_train(args):
+    """Train the closure network on every saved case (features, beta_inv -
+    1, lower half of each grid) and save its parameters with the loss
+    history."""
+    import glob
+    import json
+    import os
+    files = sorted(glob.glob(os.path.join(args.data, "case_*.npz")))
+    X = np.concatenate([np.load(f)["features"].reshape(-1, 5) for f in files]).astype(np.float32)
+    T = np.concatenate([np.load(f)["beta_inv"].reshape(-1) - 1.0 for f in files]).astype(np.float32)
+    print("training set: %d cases, %d samples" % (len(files), len(X)), flush=True)
+    net, hist, t_train = train(X, T, epochs=args.epochs, batch=65536, lr=1e-3, seed=args.net_seed)
+    print("trained %d epochs in %.2fs: loss %.3e -> %.3e" % (args.epochs, t_train, hist[0], hist[-1]), flush=True)
+    os.makedirs(os.path.dirname(os.path.abspath(args.net_out)), exist_ok=True)
+    np.savez(args.net_out, params=net.params, loss=np.asarray(hist, dtype=np.float32))
+    meta = dict(cases=len(files), samples=int(len(X)), epochs=args.epochs, batch=65536, lr=1e-3, seed=args.net_seed,
+                train_s=t_train, loss_first=float(hist[0]), loss_last=float(hist[-1]))
+    with open(os.path.splitext(args.net_out)[0] + ".json", "w") as f:
+        json.dump(meta, f, indent=1)
+    print("net", json.dumps(meta), flush=True)
+
+
+def stage_reference(args):
+    """The unseen bump (h 0.065, w 0.25, Re 5e6) on the test grid: plain SA,
+    the truth (analytic beta*) and `iterations` L-BFGS iterations of the
+    field inversion, resumed from a saved beta field when --resume is given
+    (the L-BFGS memory starts empty again).  Saves the wall Cf of all three,
+    the inverted beta and the timings."""
+    import json
+    import os
+    coef = truth_coef(seed=args.truth_seed)
+    ni, nj = (int(v) for v in args.test_grid.split("x"))
+    case = bump_case(ni, nj, 0.065, 0.25, re=5e6)
+    eq = lib.Equation(case)
+    q_sa, r_sa, t_sa = timed_solve(eq, case.freestream_state(), **NEWTON)
+    print("plain SA %.1fs (%d steps)" % (t_sa, r_sa["iterations"]), flush=True)
+    eq.set_closure(1, coef=coef)
+    q_true, r_true, t_true = timed_solve(eq, q_sa, **WARM)
+    _, _, cf_true, _ = eq.wall(q_true)
+    eq.set_closure(0)
+    _, _, cf_sa, _ = eq.wall(q_sa)
+    done = 0
+    wall_before = 0.0
+    beta0 = None
+    q0 = q_sa
+    if args.resume:
+        z = np.load(args.resume)
+        beta0, done, wall_before = z["beta_inv"], int(z["iterations_done"]), float(z["inversion_s"])
+        eq.set_beta(beta0)
+        q0, _, _ = timed_solve(eq, q_sa, **WARM)
+        print("resumed from %s after %d iterations" % (args.resume, done), flush=True)
+    t0 = time.perf_counter()
+    q_inv, beta_inv, irep, jh = eq.invert(q0, cf_true, beta=beta0, iterations=args.iterations,
+                                          newton=dict(RESOLVE, verbose=1), adjoint=ADJOINT)
+    t_inv = time.perf_counter() - t0
+    _, _, cf_inv, _ = eq.wall(q_inv)
+    eq.close()
+    os.makedirs(args.out, exist_ok=True)
+    total = done + irep["iterations"]
+    np.savez_compressed(os.path.join(args.out, "reference_%s.npz" % args.test_grid), beta_inv=beta_inv,
+                        cf_inv=cf_inv, cf_true=cf_true, cf_sa=cf_sa, J_hist=jh, iterations_done=total,
+                        inversion_s=wall_before + t_inv)
+    out = dict(test_grid=args.test_grid, sa_s=t_sa, sa_newton=r_sa["iterations"], truth_s=t_true,
+               iterations_this_run=irep["iterations"], iterations_done=total, inversion_s_this_run=t_inv,
+               inversion_s_total=wall_before + t_inv, J0=irep["J0"], J=irep["J"], evaluations=irep["evaluations"],
+               failed_solves=irep["failed_solves"], cf_err_sa_vs_truth=cf_error(cf_sa, cf_true),
+               cf_err_inv_vs_truth=cf_error(cf_inv, cf_true))
+    with open(os.path.join(args.out, "reference_%s.json" % args.test_grid), "w") as f:
+        json.dump(out, f, indent=1)
+    print("reference", json.dumps(out), flush=True)
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="field inversion and machine learning of the SA beta closure")
-    ap.add_argument("stage", choices=["invert", "test"])
+    ap.add_argument("stage", choices=["invert", "test", "train", "reference"])
     ap.add_argument("--ncases", type=int, default=24)
     ap.add_argument("--cases", default="0:24", help="slice of the training bumps to invert")
     ap.add_argument("--train-grid", default="385x193")
@@ -228,11 +300,10 @@ def main(argv=None):
     ap.add_argument("--net-seed", type=int, default=0)
     ap.add_argument("--data", default="fiml_data")
     ap.add_argument("--out", default="fiml_data")
+    ap.add_argument("--net-out", default="fiml_data/closure_net.npz")
+    ap.add_argument("--resume", default="", help="reference stage: saved reference_*.npz to continue from")
     args = ap.parse_args(argv)
-    if args.stage == "invert":
-        stage_invert(args)
-    else:
-        stage_test(args)
+    {"invert": stage_invert, "test": stage_test, "train": stage_train, "reference": stage_reference}[args.stage](args)
 
 
 if __name__ == "__main__":

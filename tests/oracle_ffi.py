@@ -128,6 +128,17 @@ def time_residual(case, q, warm=3, reps=20, threads=0):
                                   C.byref(o), abi.dptr(q), warm, reps, threads)
 
 
+def adjoint_sample(case, q, threads=0, iters=3, basis=60):
+    """(seconds to extract the Jacobian blocks, seconds per transposed
+    Krylov iteration) of the CPU baseline of the adjoint gradient."""
+    xv, yv, fs, bcs, o = _common(case)
+    q = _arr(q)
+    out = np.zeros(3)
+    lib().og_adjoint_sample(case.ni, case.nj, abi.dptr(xv), abi.dptr(yv), C.byref(fs), bcs, len(case.bcs), C.byref(o),
+                            abi.dptr(q), int(threads), int(iters), int(basis), abi.dptr(out))
+    return float(out[0]), float(out[1])
+
+
 def explicit_solve(case, q, scheme="forward_euler", cfl=1.0, iteration_max=1, cfl_ramp=False, cfl_ramp_iteration=20,
                    cfl_ramp_exponent=1.1, wd=None, beta=None):
     """Solver::solve with an explicit scheme, composed from the oracle's

@@ -132,8 +132,10 @@ def test_mlp_closure_jv_vs_oracle_dual(k):
     _, jvo = of.jv(case, q, v, wd=wd)
     contrib_o = jvo - jv_fixed
     contrib = jv - jv_fixed
-    print("closure part of J.v: max %.3e, rel err %.2e; rest rel err %.2e" % (
-        np.abs(contrib_o).max(), rel(contrib[..., 4], contrib_o[..., 4]), rel(jv[..., :4], jvo[..., :4])))
+    e = np.abs(contrib[..., 4] - contrib_o[..., 4]) / np.abs(contrib_o[..., 4]).max()
+    print("closure part of J.v: max %.3e, rel err %.2e (median %.2e p99 %.2e); rest rel err %.2e" % (
+        np.abs(contrib_o).max(), rel(contrib[..., 4], contrib_o[..., 4]), np.median(e), np.quantile(e, 0.99),
+        rel(jv[..., :4], jvo[..., :4])))
     assert np.abs(contrib_o[..., 4]).max() > 1e-4 * np.abs(jvo[..., 4]).max()
     assert rel(contrib[..., 4], contrib_o[..., 4]) <= 2e-2
     assert rel(jv[..., :4], jvo[..., :4]) <= 1e-10

@@ -72,8 +72,11 @@ def test_fullsize_jv_and_wall(name):
     assert np.abs(cf - cfo).max() <= 1e-12 * np.abs(cfo).max()
 
 
-def test_fullsize_dot_product_identity():
-    case, eq, _, _ = setup("sa_plate_545x385")
+@pytest.mark.parametrize("name", ["sa_plate_545x385", "sa_bump_705x321"])
+def test_fullsize_dot_product_identity(name):
+    """<psi, J v> = <J^T psi, v> at the SA plate and at the adjoint
+    configuration's bump (705x321 with the seeded 8-blob beta field)."""
+    case, eq, _, _ = setup(name)
     q = case.perturbed_state(seed=13)
     rng = np.random.default_rng(6)
     psi = rng.standard_normal(q.shape)
@@ -81,5 +84,45 @@ def test_fullsize_dot_product_identity():
     jt, _ = eq.jtv(q, psi)
     jv = eq.jv(q, v)
     a, b = np.vdot(psi, jv), np.vdot(jt, v)
-    print("545x385 <psi, J v> %.17e  <J^T psi, v> %.17e" % (a, b), flush=True)
+    print(name, "<psi, J v> %.17e  <J^T psi, v> %.17e" % (a, b), flush=True)
     assert abs(a - b) <= 1e-12 * np.linalg.norm(psi) * np.linalg.norm(jv)
+
+
+def test_fullsize_mlp_closure_residual_and_jv():
+    """The closure network fused into the residual and J.v kernels at the
+    largest configuration (1409x641 bump): beta against the oracle's FP64
+    network with the kernel's BF16 rounding points, the residual against the
+    oracle's at the kernel's beta (everything but the network, 1e-12), J.v
+    components 0-3 (1e-10) and the closure's part of the nu~ component."""
+    import mlp_model as mm
+    case, eq, wd, _ = setup("sa_bump_1409x641")
+    q = case.perturbed_state(seed=14)
+    p = mm.glorot(seed=4).astype(np.float32)
+    p[mm.OFF["b3"][0]] = 0.05
+    eq.set_closure(2, params=p)
+    of.set_closure(2, params=p)
+    try:
+        _, b = eq.closure_eval(q)
+        _, bo = of.closure_eval(case, q, wd=wd)
+        eb = np.abs(b - bo)
+        print("1409x641 mlp beta vs oracle: median %.2e p99.9 %.2e max %.2e" % (
+            np.median(eb), np.quantile(eb, 0.999), eb.max()), flush=True)
+        assert np.median(eb) <= 1e-6 and eb.max() <= 2e-3
+        r, _ = eq.calc_residual(q)
+        of.set_closure(0)
+        ro, _ = of.residual(case, q, wd=wd, beta=b)
+        print("1409x641 mlp residual rel err %.2e" % rel(r, ro), flush=True)
+        assert rel(r, ro) <= 1e-12
+        v = np.random.default_rng(15).standard_normal(q.shape) * np.abs(q).max(axis=(0, 1))
+        jv = eq.jv(q, v)
+        _, jv_fixed = of.jv(case, q, v, wd=wd, beta=b)
+        of.set_closure(2, params=p)
+        _, jvo = of.jv(case, q, v, wd=wd)
+        co, cg = (jvo - jv_fixed)[..., 4], (jv - jv_fixed)[..., 4]
+        e = np.abs(cg - co) / np.abs(co).max()
+        print("1409x641 closure part of J.v: median %.2e p99.9 %.2e max %.2e; rest %.2e" % (
+            np.median(e), np.quantile(e, 0.999), e.max(), rel(jv[..., :4], jvo[..., :4])), flush=True)
+        assert rel(jv[..., :4], jvo[..., :4]) <= 1e-10
+        assert np.median(e) <= 1e-6 and e.max() <= 2e-2
+    finally:
+        of.set_closure(0)

@@ -115,28 +115,40 @@ def test_ilu_matches_numpy_model_sa(precond):
 
 @pytest.mark.parametrize("width,sa", [(8, False), (16, False), (8, True), (32, True)])
 @pytest.mark.parametrize("transpose", [False, True])
-def test_chunked_ilu_matches_numpy_model(width, sa, transpose):
-    """Chunked line ILU(0) (precond 6): chunks of `width` columns (a partial
-    last chunk at width 8 on 12 columns, and at width 32 on 40 columns),
-    M^-1 and M^-T against the model with float32 factors."""
+@pytest.mark.parametrize("precond", [6, 7])
+def test_chunked_ilu_matches_numpy_model(width, sa, transpose, precond):
+    """Chunked line ILU(0) (precond 6; precond 7 on the 3x3-box pattern):
+    chunks of `width` columns (a partial last chunk at width 8 on 12 columns,
+    and at width 32 on 40 columns), M^-1 and M^-T against the model with
+    float32 factors.  precond 6 factors the exact Jacobian; precond 7, meant
+    for lhs_order 1, the first-order one (whose +-2 blocks are zero, so the
+    box pattern drops fill only)."""
+    import copy
     import precond_model as pm
     ni = 41 if width == 32 else 13
     case = cases.sa_bump(ni, 9, dy1=1e-3) if sa else cases.laminar_plate(ni, 9, dy1=1e-2)
     q = case.perturbed_state(seed=6, amp=0.01)
     eq = lib.Equation(case)
     dt = eq.calc_dt(q, 50.0)
-    eq.ls_preallocate(restart=10, precond=6, maxit=10, rtol=1e-6)
-    eq.ls_set_pc(0, 1.0, width)
+    nine = precond == 7
+    eq.ls_preallocate(restart=10, precond=precond, maxit=10, rtol=1e-6)
+    eq.ls_set_pc(1 if nine else 0, 1.0, width)
     eq.ls_set_lhs(q, dt)
     v = np.random.default_rng(3).standard_normal(q.shape)
     z = eq.ls_precond_apply(v, transpose=transpose)
-    J = of.jacobian(case, q, wd=eq.walldist() if sa else None)
+    cj = copy.deepcopy(case)
+    if nine:
+        cj.order = 1
+    J = of.jacobian(cj, q, wd=eq.walldist() if sa else None)
     B = pm.blocks_from_dense(J, case.nic, case.njc, case.nvar)
-    zr = pm.ilu_chunked(B, 1.0 / dt[..., 0], v, width, transpose=transpose)
-    z64 = pm.ilu_chunked(B, 1.0 / dt[..., 0], v, width, transpose=transpose, f32=False)
+    zr = pm.ilu_chunked(B, 1.0 / dt[..., 0], v, width, transpose=transpose, nine_point=nine)
+    z64 = pm.ilu_chunked(B, 1.0 / dt[..., 0], v, width, transpose=transpose, f32=False, nine_point=nine)
     err = float(np.abs(z - zr).max() / np.abs(zr).max())
-    print("chunked ILU width", width, "sa", sa, "transpose", transpose, "model err", err,
+    print("chunked ILU precond", precond, "width", width, "sa", sa, "transpose", transpose, "model err", err,
           "(float64 factors: %.1e)" % float(np.abs(z - z64).max() / np.abs(z64).max()))
+    if nine:  # the 13-point model (which keeps the fill) misses by far more than the bar
+        zo = pm.ilu_chunked(B, 1.0 / dt[..., 0], v, width, transpose=transpose)
+        assert float(np.abs(z - zo).max() / np.abs(zo).max()) > 1e-7
     # the model stores its factors in float32 like the device; rounding at a
     # float32 tie can differ in the last place, far below 1e-8
     assert err <= 1e-8

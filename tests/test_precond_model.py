@@ -197,6 +197,43 @@ def test_nine_point_chunked_ilu_is_exact_on_one_tridiagonal_column():
     assert np.abs(z - full).max() > 1e-4 * np.abs(full).max()
 
 
+def test_nine_point_chunked_ilu_is_exact_on_one_row_in_a_chunk():
+    """The mirror case along i: one row of cells (njc = 1) inside one chunk,
+    ordered by i.  With only the i +- 1 couplings the matrix is block
+    tridiagonal in that order, its LU has no fill and the box-pattern model
+    is A^-1; with the (+-2, 0) couplings present it is not (they lie outside
+    the 3x3 box, so the keep rule along i drops them)."""
+    nic, njc, nv = 8, 1, 5
+    B, idt = _random_blocks(nic, njc, nv, seed=21)
+    v = np.random.default_rng(22).standard_normal((nic, njc, nv))
+    Bt = _keep_slots(B, [0, 3, 4])
+    for tr in (False, True):
+        A = _dense(Bt, idt)
+        ref = np.linalg.solve(A.T if tr else A, v.reshape(-1)).reshape(v.shape)
+        z = pm.ilu_chunked(Bt, idt, v, 8, transpose=tr, f32=False, nine_point=True)
+        assert np.abs(z - ref).max() <= 1e-12 * np.abs(ref).max(), tr
+    Bw = _keep_slots(B, [0, 3, 4, 11, 12])
+    full = _exact(Bw, idt, v)
+    z = pm.ilu_chunked(Bw, idt, v, 8, f32=False, nine_point=True)
+    assert np.abs(z - full).max() > 1e-4 * np.abs(full).max()
+    # the 13-point pattern keeps them and is exact again
+    z13 = pm.ilu_chunked(Bw, idt, v, 8, f32=False)
+    assert np.abs(z13 - full).max() <= 1e-12 * np.abs(full).max()
+
+
+def test_float16_scaled_known_answer():
+    """A block scaled by its largest magnitude 3: the entry 1 is stored as
+    the float16 nearest to 1/3, 0x3555 = 0.333251953125, and read back as
+    3 x that = 0.999755859375 (float16 without the scale would keep 1 exact);
+    the largest entry and entries exact after scaling (1.5 -> 0.5, -0.75 ->
+    -0.25) come back unchanged; an all-zero block stays zero."""
+    blk = np.array([[3.0, 1.0], [1.5, -0.75]])
+    out = pm.f16_scaled(blk)
+    assert out[0, 1] == 0.999755859375
+    assert out[0, 0] == 3.0 and out[1, 0] == 1.5 and out[1, 1] == -0.75
+    assert np.array_equal(pm.f16_scaled(np.zeros((2, 2))), np.zeros((2, 2)))
+
+
 def test_float16_scaled_factors_round_at_float16_level():
     """float16 storage with a scale per block moves the result by about
     float16 rounding (2^-11 relative per entry), far more than float32."""
