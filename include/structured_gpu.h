@@ -194,6 +194,15 @@ int sg_jtv_dev(sg_ctx* ctx, const double* q_dev, const double* psi_dev, double* 
  * ||R_k||_2 into l2_dev[0..nvar-1], device memory: exactly nvar doubles are
  * written) without synchronising. */
 int sg_residual_dev_async(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_dev);
+/* sg_residual_dev_async without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  The norm scratch (per-tile partial sums and
+ * an arrival counter) lives in a static device array of each GPU, at most
+ * SG_NOALLOC_MAX_TILES residual tiles of 31 x 7 cells (1.25 MiB, held until
+ * the process ends); a larger grid is SG_ERR_ARG.  Calls that overlap on one
+ * GPU share that array and race: serialise them. */
+#define SG_NOALLOC_MAX_TILES 16384
+int sg_residual_dev_async_noalloc(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_dev);
 int sg_synchronize(sg_ctx* ctx);
 /* The context's HIP stream (a hipStream_t), so callers can order their own
  * work and events with the library's. */

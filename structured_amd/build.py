@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libstructured_gpu.so")
 SOURCES = ["sg_api.cpp", "sg_kernels.hip", "sg_linalg.hip", "sg_solver.cpp", "sg_mlp.hip",
            "sg_mlp_api.cpp", "sg_inversion.cpp", "sg_cilu.hip", "sg_explicit.hip",
-           "sg_io.cpp"]
+           "sg_io.cpp", "sg_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-fopenmp"]

@@ -370,6 +370,7 @@ int sg_create(int ni, int nj, const double* xv, const double* yv, const sg_frees
     return code;
   };
   if (hipStreamCreate(&ctx->stream) != hipSuccess) return fail(SG_ERR_HIP);
+  if ((rc = sg_noalloc_setup(ctx))) return fail(rc);
   // device geometry
   const int ldi = ((G.nic + 63) / 64) * 64, ldf = ((G.ni + 63) / 64) * 64;
   DevGeom& g = ctx->g;

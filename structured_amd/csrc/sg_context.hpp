@@ -85,10 +85,12 @@ struct sg_ctx {
   sg::DevBuf aux;                  // features (5 planes) + beta, for sg_closure_eval
   sg::DevBuf ex_a, ex_b, ex_r;     // explicit stepping: stage states and residual
   SolverState* solver = nullptr;   // Newton-Krylov work space (sg_solver.cpp)
+  double* noalloc_scratch = nullptr;  // this device's static scratch (sg_noalloc.hip)
   size_t ncell() const { return (size_t)geo.nic * geo.njc; }
   size_t nsoa() const { return (size_t)nvar * g.plane; }
 };
 
 int sg_fail(sg_ctx* ctx, int code, const std::string& msg);
+int sg_noalloc_setup(sg_ctx* ctx);
 void sg_solver_free(sg_ctx* ctx);
 int sg_hip(sg_ctx* ctx, hipError_t e, const char* what);
