@@ -142,9 +142,53 @@ def adjoint_leg(grid, cpu=True):
            "primal_cold_solve_s": t_cold, "primal_cold_newton": r1["iterations"],
            "primal_warm_solve_s": t_warm, "primal_converged": [bool(r["converged"]) for r in (r1, r2, r3)],
            "inversion_iteration_s_estimate": t_warm + t_adj}
+    out["preconditioner_apply"] = ilu_apply_leg(eq, q, adj)
     eq.close()
     if cpu:
         out["cpu_baseline"] = adjoint_cpu_baseline(case, q, rep["iterations"])
+    return out
+
+
+def ilu_apply_leg(eq, q, adj, applies=50):
+    """Per-apply time of the adjoint's preconditioner (chunked line ILU(0)
+    of the first-order Jacobian at the adjoint's pc CFL, chunk width 32):
+    M^-1 and M^-T on device vectors, back to back on the library's stream,
+    timed with HIP events on that stream, and the factorisation (blocks by
+    coloured J.v sweeps, gather, factor) timed by the host around set_lhs."""
+    import ctypes as C
+    import torch
+    from structured_amd import lib
+    so = lib.load()
+    dev = torch.device("cuda:0")
+    dt = eq.calc_dt(q, adj["pc_cfl"])
+    eq.ls_preallocate(restart=10, precond=adj["precond"], maxit=10, rtol=1e-6)
+    eq.ls_set_pc(adj["lhs_order"], 1.0, 32)
+    eq.ls_set_lhs(q, dt)
+    t0 = time.perf_counter()
+    eq.ls_set_lhs(q, dt)
+    t_setup = time.perf_counter() - t0
+    n = eq.nvar * eq.plane
+    vs = [torch.randn(n, dtype=torch.float64, device=dev) for _ in range(2)]
+    zs = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(2)]
+    stream = torch.cuda.ExternalStream(eq.stream(), device=dev)
+    out = {"precond": adj["precond"], "lhs_order": adj["lhs_order"], "pc_cfl": adj["pc_cfl"], "width": 32,
+           "set_lhs_ms": 1e3 * t_setup, "applies": applies}
+    for name, tr in (("apply_inverse_us", 0), ("apply_inverse_transpose_us", 1)):
+        def run(k_):
+            for k in range(k_):
+                rc = so.sg_ls_precond_apply_dev_async(eq.h, C.c_void_p(vs[k % 2].data_ptr()),
+                                                      C.c_void_p(zs[k % 2].data_ptr()), tr)
+                assert rc == 0, so.sg_last_error(eq.h)
+        run(5)
+        eq.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        run(applies)
+        e1.record(stream)
+        eq.synchronize()
+        out[name] = 1e3 * e0.elapsed_time(e1) / applies
+    del vs, zs
+    torch.cuda.empty_cache()
     return out
 
 

@@ -235,6 +235,12 @@ int sg_ls_solve_and_update(sg_ctx* ctx, double* q_aos, double under_relaxation);
 /* z = M^-1 v (transpose = 0) or M^-T v (transpose = 1, preconditioners 0,
  * 4, 5, 6 and 7) with the preconditioner built by the last set_lhs. */
 int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos, int transpose);
+/* sg_ls_precond_apply on device vectors in the library's SoA layout (as
+ * sg_residual_dev), enqueued on the context's stream without synchronising;
+ * v_dev and z_dev must not overlap.  The preconditioner application inside
+ * LinearSolver::solve (src/solver/linearsolver.cpp, the PETSc PC), exposed
+ * for callers that keep their Krylov vectors on the device. */
+int sg_ls_precond_apply_dev_async(sg_ctx* ctx, const double* v_dev, double* z_dev, int transpose);
 
 /* ---- nonlinear solve: Solver::solve / Solver::step, src/solver/solver.cpp:52-236.
  *      Ramp semantics as the reference (after step `counter`, if
@@ -362,9 +368,15 @@ typedef struct sg_inversion_opts {
   sg_adjoint_opts adjoint;
 } sg_inversion_opts;
 typedef struct sg_inversion_report {
-  int iterations, evaluations, failed_solves, reserved;
+  int iterations, evaluations, failed_solves;
+  int adjoint_unconverged;  /* adjoint solves that stopped above rtol (their
+                               gradient is used all the same)                */
   double J0, J, F0, F;
   double wall_s, primal_s, adjoint_s;
+  double adjoint_relres_max;  /* largest final relres of the adjoint solves  */
+  int adjoint_iterations;     /* Krylov iterations of all adjoint solves     */
+  int steepest_restarts;      /* line searches retried along -g after the
+                                 L-BFGS direction found no Armijo point      */
 } sg_inversion_report;
 /* q_aos: converged state at the initial beta (in), state at the final beta
  * (out); beta [nic][njc] in/out; J_hist (iterations + 1 entries, may be NULL)

@@ -116,6 +116,8 @@ class InversionOpts(C.Structure):
 
 
 class InversionReport(C.Structure):
-    _fields_ = [("iterations", C.c_int), ("evaluations", C.c_int), ("failed_solves", C.c_int), ("reserved", C.c_int),
+    _fields_ = [("iterations", C.c_int), ("evaluations", C.c_int), ("failed_solves", C.c_int),
+                ("adjoint_unconverged", C.c_int),
                 ("J0", C.c_double), ("J", C.c_double), ("F0", C.c_double), ("F", C.c_double),
-                ("wall_s", C.c_double), ("primal_s", C.c_double), ("adjoint_s", C.c_double)]
+                ("wall_s", C.c_double), ("primal_s", C.c_double), ("adjoint_s", C.c_double),
+                ("adjoint_relres_max", C.c_double), ("adjoint_iterations", C.c_int), ("steepest_restarts", C.c_int)]

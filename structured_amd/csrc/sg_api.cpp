@@ -547,7 +547,8 @@ int sg_residual_dev_async(sg_ctx* ctx, const double* q_dev, double* r_dev, doubl
   if (!ctx || !q_dev || !r_dev) return SG_ERR_ARG;
   // the caller's l2_dev holds nvar norms (||R_k||_2): only those are written
   SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, l2_dev ? ctx->partial.p : nullptr,
-                         reinterpret_cast<unsigned long long*>(ctx->counter.p), l2_dev, ctx->stream, ctx->clo, ctx->nvar),
+                         reinterpret_cast<unsigned long long*>(ctx->counter.p), l2_dev, ctx->stream, ctx->clo, ctx->nvar,
+                         &ctx->norm_parity),
          "residual launch");
   return SG_OK;
 }
@@ -557,7 +558,8 @@ void* sg_get_stream(sg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 int sg_residual_dev(sg_ctx* ctx, const double* q_dev, double* r_dev, double* l2_host) {
   if (!ctx || !q_dev || !r_dev) return SG_ERR_ARG;
   SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, ctx->partial.p,
-                         reinterpret_cast<unsigned long long*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
+                         reinterpret_cast<unsigned long long*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo, 0,
+                         &ctx->norm_parity),
          "residual launch");
   double l2[8];
   SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, ctx->nvar * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
@@ -572,7 +574,8 @@ int sg_residual(sg_ctx* ctx, const double* q_aos, double* r_aos, double* l2) {
   if ((rc = upload_aos(ctx, q_aos, ctx->q.p))) return rc;
   double l2h[8];
   SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, ctx->q.p, ctx->r.p, ctx->partial.p,
-                         reinterpret_cast<unsigned long long*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
+                         reinterpret_cast<unsigned long long*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo, 0,
+                         &ctx->norm_parity),
          "residual launch");
   SG_TRY(hipMemcpyAsync(l2h, ctx->l2.p, ctx->nvar * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
   if ((rc = download_aos(ctx, ctx->r.p, r_aos))) return rc;

@@ -78,7 +78,8 @@ struct sg_ctx {
   sg::DevBuf nodes_buf, map_buf;   // BcNode / int storage (raw doubles)
   sg::DevBuf aos_a, aos_b;         // AoS staging (nvar * ncell)
   sg::DevBuf q, r, v, w;           // SoA work vectors (nvar * plane)
-  sg::DevBuf partial, l2, counter;  // counter: one unsigned, kept at 0 between launches
+  sg::DevBuf partial, l2, counter;  // norm partials; counter: the arrival count, 0 between launches
+  unsigned norm_parity = 0;         // parity of the next fused norm on `partial`
   sg::DevBuf wall_cp, wall_cf, wall_f;
   sg::DevClosure clo;              // beta closure applied by every residual / J.v launch
   sg::DevBuf clo_params, clo_w2;   // MLP parameters (FP32) and W2 (BF16), raw storage

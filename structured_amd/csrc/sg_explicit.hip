@@ -73,7 +73,7 @@ int enqueue_step(sg_ctx* ctx, double* q, int scheme, double cfl) {
   double* R = ctx->ex_r.p;
   auto residual = [&](const double* in) {
     return launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, in, R, ctx->partial.p, counter, ctx->l2.p, ctx->stream,
-                           ctx->clo);
+                           ctx->clo, 0, &ctx->norm_parity);
   };
   if (scheme == SG_SCHEME_FORWARD_EULER) {
     SG_TRY(residual(q), "residual");

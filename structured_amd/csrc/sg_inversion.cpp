@@ -64,6 +64,12 @@ extern "C" int sg_invert(sg_ctx* ctx, double* q_aos, const double* cf_target, do
     int rc = sg_adjoint_gradient(ctx, qs.data(), cf_target, ga.data(), F, &o->adjoint, &ar);
     t_adjoint += c.s();
     if (rc != SG_OK && rc != SG_ERR_NOCONV) return rc;
+    R.adjoint_unconverged += rc == SG_ERR_NOCONV ? 1 : 0;
+    R.adjoint_relres_max = std::max(R.adjoint_relres_max, ar.relres);
+    R.adjoint_iterations += ar.iterations;
+    if (o->newton.verbose)
+      std::printf("adjoint   %d iterations, relres %.3e%s, F %.6e\n", ar.iterations, ar.relres,
+                  rc == SG_ERR_NOCONV ? " (not converged)" : "", *F);
     for (size_t k = 0; k < nc; ++k) g[k] = ga[k] + 2.0 * lam * (b[k] - 1.0);
     return SG_OK;
   };
@@ -114,21 +120,36 @@ extern "C" int sg_invert(sg_ctx* ctx, double* q_aos, const double* cf_target, do
       for (size_t k = 0; k < nc; ++k) d[k] = -g[k] * (o->step0 / gmax);
       gd = dot(g, d);
     }
-    // projected Armijo backtracking
-    double a = 1.0, Jn = 0.0, Fn = 0.0;
+    // projected Armijo backtracking; if the L-BFGS direction finds no
+    // acceptable point, the search is repeated once along scaled -g with the
+    // correction pairs dropped, and only a failure along -g ends the run
+    double Jn = 0.0, Fn = 0.0;
     bool ok = false;
-    for (int bt = 0; bt <= o->max_backtracks; ++bt, a *= 0.25) {
-      for (size_t k = 0; k < nc; ++k) xn[k] = std::min(o->beta_max, std::max(o->beta_min, x[k] + a * d[k]));
-      qn = q;
-      int pr = primal(xn, qn, &sr);
-      ++R.evaluations;
-      if (pr == SG_ERR_NAN || pr == SG_ERR_NOCONV) { ++R.failed_solves; continue; }
-      if (pr != SG_OK) return pr;
-      if ((rc = sg_objective_cf(ctx, qn.data(), cf_target, &Fn, nullptr))) return rc;
-      Jn = Fn + prior(xn);
-      double gs = 0.0;
-      for (size_t k = 0; k < nc; ++k) gs += g[k] * (xn[k] - x[k]);
-      if (Jn <= J + 1e-4 * gs) { ok = true; break; }
+    for (int attempt = 0; attempt < 2 && !ok; ++attempt) {
+      if (attempt == 1) {
+        if (S.empty()) break;  // the direction was -g already
+        S.clear(); Y.clear(); RHO.clear();
+        double gmax = 0.0;
+        for (double v : g) gmax = std::max(gmax, std::fabs(v));
+        for (size_t k = 0; k < nc; ++k) d[k] = -g[k] * (o->step0 / gmax);
+        ++R.steepest_restarts;
+      }
+      double a = 1.0;
+      for (int bt = 0; bt <= o->max_backtracks; ++bt, a *= 0.25) {
+        for (size_t k = 0; k < nc; ++k) xn[k] = std::min(o->beta_max, std::max(o->beta_min, x[k] + a * d[k]));
+        qn = q;
+        int pr = primal(xn, qn, &sr);
+        ++R.evaluations;
+        if (pr == SG_ERR_NAN || pr == SG_ERR_NOCONV) { ++R.failed_solves; continue; }
+        if (pr != SG_OK) return pr;
+        if ((rc = sg_objective_cf(ctx, qn.data(), cf_target, &Fn, nullptr))) return rc;
+        Jn = Fn + prior(xn);
+        double gs = 0.0;
+        for (size_t k = 0; k < nc; ++k) gs += g[k] * (xn[k] - x[k]);
+        if (o->newton.verbose)
+          std::printf("linesearch step %.3e  J %.9e  (J0 %.9e, Armijo bound %.9e)\n", a, Jn, J, J + 1e-4 * gs);
+        if (Jn <= J + 1e-4 * gs) { ok = true; break; }
+      }
     }
     if (!ok) break;
     std::vector<double> gold = g;

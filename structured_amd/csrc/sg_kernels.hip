@@ -20,7 +20,7 @@ size_t residual_partial_count(const DevGeom& g) {
 template <class S, bool SA>
 static void launch_cl(dim3 grid, hipStream_t s, const DevGeom& g, const DevPhys& ph, const DevBc& bc, Field<S> fq,
                       OutField<S> fo, double* partial, const double* beta_dot, const DevClosure& clo,
-                      NormOut no = NormOut{nullptr, nullptr, 0}) {
+                      NormOut no = NormOut{nullptr, nullptr, 0, 0u}) {
   if constexpr (SA) {
     if (clo.kind == CLOSURE_ANALYTIC) {
       hipLaunchKernelGGL((residual_kernel<S, SA, CLOSURE_ANALYTIC>), grid, dim3(NT), 0, s, g, ph, bc, fq, fo, partial,
@@ -39,7 +39,7 @@ static void launch_cl(dim3 grid, hipStream_t s, const DevGeom& g, const DevPhys&
 
 hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, double* r,
                            double* partial, unsigned long long* counter, double* l2, hipStream_t s, const DevClosure& clo,
-                           int l2_count) {
+                           int l2_count, unsigned* parity) {
   dim3 grid = tile_grid(g);
   Field<double> fq{q};
   OutField<double> fo{r};
@@ -47,11 +47,15 @@ hipError_t launch_residual(const DevGeom& g, const DevPhys& ph, const DevBc& bc,
   if (l2_count <= 0 || l2_count > nv2) l2_count = nv2;
   // the norms are finalised inside the residual kernel; partials without a
   // state pair and a norm buffer are not a supported combination
-  if ((l2 != nullptr) != (partial != nullptr && counter != nullptr) || (partial && !l2)) return hipErrorInvalidValue;
-  const NormOut no{counter, l2, l2_count};
+  if ((l2 != nullptr) != (partial != nullptr && counter != nullptr && parity != nullptr) || (partial && !l2))
+    return hipErrorInvalidValue;
+  const NormOut no{counter, l2, l2_count, parity ? *parity : 0u};
   if (sa) launch_cl<double, true>(grid, s, g, ph, bc, fq, fo, partial, nullptr, clo, no);
   else launch_cl<double, false>(grid, s, g, ph, bc, fq, fo, partial, nullptr, DevClosure(), no);
-  return hipGetLastError();
+  const hipError_t e = hipGetLastError();
+  // the next fused launch on these partials uses the other parity
+  if (e == hipSuccess && l2) *parity ^= 1u;
+  return e;
 }
 
 hipError_t launch_jv(const DevGeom& g, const DevPhys& ph, const DevBc& bc, bool sa, const double* q, const double* v,

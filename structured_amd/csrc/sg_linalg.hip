@@ -13,12 +13,18 @@ namespace sg {
 constexpr int DOT_BLOCK = 256;
 constexpr int DOT_GRID = 2048;  // partial sums per dot product (8 blocks per CU)
 
+// Gate of the conditional kernels: with ga and gb (device) the kernel runs
+// only when *ga < *gb / 2, else it returns at once (nothing written).
+__device__ __forceinline__ bool gate_open(const double* ga, const double* gb) { return !ga || *ga < 0.5 * *gb; }
+
 // out_partial[b*k + i] = sum over this block's slice of V_i . w, for groups
 // of DOT_KC vectors at a time (accumulators in registers), each summed over
 // the block in a fixed order (DPP wavefront sums, then the four waves).
 constexpr int DOT_KC = 8;
 __global__ __launch_bounds__(DOT_BLOCK) void multi_dot_kernel(const double* V, size_t ldv, int k, const double* w,
-                                                              size_t n, double* partial) {
+                                                              size_t n, double* partial, const double* ga,
+                                                              const double* gb) {
+  if (!gate_open(ga, gb)) return;
   __shared__ double red[DOT_BLOCK / 64][DOT_KC];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int i0 = 0; i0 < k; i0 += DOT_KC) {
@@ -50,7 +56,8 @@ __global__ __launch_bounds__(DOT_BLOCK) void multi_dot_kernel(const double* V, s
 
 // out[i] = sum_b partial[b*k + i]  (+ optional add into out when accumulate)
 __global__ __launch_bounds__(256) void dot_finalize_kernel(const double* partial, int nb, int k, double* out,
-                                                           int accumulate) {
+                                                           int accumulate, const double* ga, const double* gb) {
+  if (!gate_open(ga, gb)) return;
   const int i = blockIdx.x;
   __shared__ double red[4];
   double s = 0.0;
@@ -65,16 +72,19 @@ __global__ __launch_bounds__(256) void dot_finalize_kernel(const double* partial
 }
 
 hipError_t multi_dot(const double* V, size_t ldv, int k, const double* w, size_t n, double* partial, double* out,
-                     bool accumulate, hipStream_t s) {
-  hipLaunchKernelGGL(multi_dot_kernel, dim3(DOT_GRID), dim3(DOT_BLOCK), 0, s, V, ldv, k, w, n, partial);
-  hipLaunchKernelGGL(dot_finalize_kernel, dim3(k), dim3(256), 0, s, partial, DOT_GRID, k, out, accumulate ? 1 : 0);
+                     bool accumulate, hipStream_t s, const double* ga, const double* gb) {
+  hipLaunchKernelGGL(multi_dot_kernel, dim3(DOT_GRID), dim3(DOT_BLOCK), 0, s, V, ldv, k, w, n, partial, ga, gb);
+  hipLaunchKernelGGL(dot_finalize_kernel, dim3(k), dim3(256), 0, s, partial, DOT_GRID, k, out, accumulate ? 1 : 0, ga,
+                     gb);
   return hipGetLastError();
 }
 
 size_t multi_dot_partial_size(int kmax) { return (size_t)DOT_GRID * kmax; }
 
 // w -= sum_i c[i] V_i   (c in device memory)
-__global__ void multi_axpy_kernel(const double* V, size_t ldv, int k, const double* c, double* w, size_t n) {
+__global__ void multi_axpy_kernel(const double* V, size_t ldv, int k, const double* c, double* w, size_t n,
+                                  const double* ga, const double* gb) {
+  if (!gate_open(ga, gb)) return;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     double s = w[e];
     for (int i = 0; i < k; ++i) s -= c[i] * V[(size_t)i * ldv + e];
@@ -82,8 +92,9 @@ __global__ void multi_axpy_kernel(const double* V, size_t ldv, int k, const doub
   }
 }
 
-hipError_t multi_axpy(const double* V, size_t ldv, int k, const double* c, double* w, size_t n, hipStream_t s) {
-  hipLaunchKernelGGL(multi_axpy_kernel, dim3(2048), dim3(256), 0, s, V, ldv, k, c, w, n);
+hipError_t multi_axpy(const double* V, size_t ldv, int k, const double* c, double* w, size_t n, hipStream_t s,
+                      const double* ga, const double* gb) {
+  hipLaunchKernelGGL(multi_axpy_kernel, dim3(2048), dim3(256), 0, s, V, ldv, k, c, w, n, ga, gb);
   return hipGetLastError();
 }
 

@@ -18,16 +18,17 @@
 
 namespace {
 
-// Layout: [0, 32) doubles hold the fused norm's two state words (start
-// ticket count, launch parity; the rest keeps the partials 256-byte
-// aligned); then the partials, 2 nvar per residual tile, component-major.
+// Layout: [0, 32) doubles hold the fused norm's arrival counter (one 64-bit
+// word; the rest keeps the partials 256-byte aligned); then the partials,
+// 2 nvar per residual tile, component-major.
 constexpr size_t kCounterDoubles = 32;
 constexpr size_t kScratchDoubles = kCounterDoubles + (size_t)SG_NOALLOC_MAX_TILES * 10;
 
 static __device__ __align__(256) double scratch[kScratchDoubles];
 
-// Zeroes the state words and the n partials the next launch uses: another
-// call may have left them from a grid with another tile count.
+// Zeroes the counter and the n partials the next launch uses: another call
+// may have left them from a grid with another tile count.  Each call then
+// starts at parity 0.
 __global__ void clear_scratch_kernel(double* s, size_t n) {
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
     s[k] = 0.0;
@@ -68,15 +69,16 @@ extern "C" int sg_residual_dev_async_noalloc(sg_ctx* ctx, const double* q_dev, d
   if (l2_dev) {
     state = reinterpret_cast<unsigned long long*>(ctx->noalloc_scratch);
     partial = ctx->noalloc_scratch + kCounterDoubles;
-    // state and partials must start at zero; a kernel, not a memset, so that
-    // the call can be captured into a graph
+    // counter and partials must start at zero (parity 0); a kernel, not a
+    // memset, so that the call can be captured into a graph
     const size_t n = kCounterDoubles + tiles * 2 * (size_t)ctx->nvar;
     hipLaunchKernelGGL(clear_scratch_kernel, dim3((unsigned)std::min<size_t>((n + 255) / 256, 1024)), dim3(256), 0,
                        ctx->stream, ctx->noalloc_scratch, n);
     if (hipGetLastError() != hipSuccess) return sg_fail(ctx, SG_ERR_HIP, "scratch clear launch");
   }
+  unsigned parity = 0;
   const hipError_t e = sg::launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q_dev, r_dev, partial, state, l2_dev,
-                                           ctx->stream, ctx->clo, ctx->nvar);
+                                           ctx->stream, ctx->clo, ctx->nvar, &parity);
   if (e != hipSuccess) return sg_fail(ctx, SG_ERR_HIP, std::string("residual launch: ") + hipGetErrorString(e));
   return SG_OK;
 }

@@ -350,13 +350,14 @@ __device__ __forceinline__ SaTerms<S> sa_terms(S r, S n, S ml, const S* gu, cons
   return t;
 }
 
-// Fused norm output: with l2 set, the block that starts last finalises the
-// norms.  state[0] counts the blocks that have started the launch (0 between
-// launches); state[1] is the launch parity, flipped by every fused launch.
+// Fused norm output: with l2 set, the block that finishes last finalises the
+// norms.  *state counts the blocks that have finished (0 between launches);
+// parity alternates between consecutive fused launches on the same partials.
 struct NormOut {
   unsigned long long* state;
   double* l2;         // ||R_k||_2 (k < NV), then ||V R_k||_2 (NV <= k < 2 NV)
   int count;          // entries of l2 written: NV (caller's buffer) or 2 NV
+  unsigned parity;
 };
 
 // Spin budget of the finalising block per thread (polls beyond the first
@@ -394,8 +395,6 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
   const int nic = g.nic, njc = g.njc;
   const double nan = __builtin_nan("");
   const bool fused = partial && nout.l2;
-  // [0]: this block's start ticket, [1]: the launch parity (fused norm only)
-  __shared__ unsigned long long ticket[2];
 
   // ---- stage primitives (+ transport properties) of tile + halo in LDS.
   // A thread stages at most NPASS positions; the conservative values of its
@@ -416,13 +415,6 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
 #pragma unroll
       for (int k = 0; k < NQ; ++k) qraw[pass][k] = q[k * g.plane + idx];
     }
-  }
-  // Fused norm: take a start ticket while the tile's loads are in flight.
-  // The block holding the last ticket finalises the norms: every other block
-  // has started by then, so its wait for their partials always ends.
-  if (fused && tid == 0) {
-    ticket[0] = __hip_atomic_fetch_add(nout.state, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ticket[1] = __hip_atomic_load(nout.state + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 #pragma unroll
   for (int pass = 0; pass < NPASS; ++pass) {
@@ -680,6 +672,13 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
       sq[NV + k] = sq[k] * V * V;
     }
   }
+  // Fused norm: count this block's arrival before its stores.  Vector memory
+  // operations complete in issue order, so a wait for the count's return
+  // issued after the stores would also wait for them to be acknowledged; the
+  // parity protocol below makes the order of arrival and stores irrelevant.
+  unsigned long long arrival = 0;
+  if (fused && tid == 0)
+    arrival = __hip_atomic_fetch_add(nout.state, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (owns) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) out.put(k * g.plane + oidx, res[k]);
@@ -698,23 +697,28 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
     // Fused: the sign bit of each stored partial (a sum of squares, >= 0)
     // carries the launch parity -- set on launches of parity 0 -- so the
     // finalising block recognises this launch's value in every slot from
-    // that slot alone.  No ordering between slots, or between a slot and the
-    // ticket, is needed: each slot is one 64-bit atomic store, read by
-    // atomic loads of the same location, and launches on a stream are
-    // ordered by the kernel boundary.  Slots start at +0.0 (parity 1), so a
-    // slot never written in this context never passes for parity 0.
-    const unsigned long long par = fused ? (ticket[1] & 1ull) : 1ull;
+    // that slot alone.  A block therefore counts its arrival without waiting
+    // for its partial stores, and no ordering between slots, or between a
+    // slot and the counter, is needed: each slot is one 64-bit atomic store,
+    // read by atomic loads of the same location, and launches on a stream
+    // are ordered by the kernel boundary.  Slots start at +0.0 (parity 1), so
+    // a slot never written in this context never passes for parity 0.
+    const bool want_neg = fused && (nout.parity & 1u) == 0u;
+    __shared__ bool last;
     if (tid < 2 * NV) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
-      if (par == 0) s = -s;
+      if (want_neg) s = -s;
       __hip_atomic_store(&partial[(size_t)tid * nb + bid], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (fused && ticket[0] == (unsigned long long)(nb - 1)) {
-      // The block that started last sums the partials of all blocks in a
+    if (fused && tid == 0) last = arrival == (unsigned long long)(nb - 1);
+    if (fused) __syncthreads();
+    if (fused && last) {
+      // The block that arrives last sums the partials of all blocks in a
       // fixed order (so the norm does not depend on which block it is),
-      // polling each slot until it holds this launch's value.
-      const bool want_neg = par == 0;
+      // polling each slot until it holds this launch's value; every other
+      // block has arrived, so it is running and stores its partials, and the
+      // wait ends.
       int spins = 0;
       double acc[2 * NV];
 #pragma unroll
@@ -757,10 +761,7 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
         for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
         nout.l2[tid] = sqrt(s);
       }
-      if (tid == 0) {  // ready for the next launch: no ticket taken, parity flipped
-        __hip_atomic_store(nout.state, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(nout.state + 1, par ^ 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      if (tid == 0) __hip_atomic_store(nout.state, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }

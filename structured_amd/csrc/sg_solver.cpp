@@ -62,6 +62,7 @@ struct SolverState {
   int maxit = 60;
   int last_iters = 0;
   double last_relres = 0.0;
+  long reorth_count = 0;     // Arnoldi steps that took the second Gram-Schmidt pass
   std::vector<double> l2_hist, cfl_hist, ur_hist;
   int nhist = 0;
 };
@@ -272,18 +273,32 @@ int fgmres(sg_ctx* ctx, double rtol, int maxit, int* iters, double* relres) {
       double* W = S->V.p + (size_t)(j + 1) * n;
       if ((rc = precond_apply(ctx, Vj, Zj))) return rc;
       if ((rc = apply_operator(ctx, Zj, W))) return rc;
-      // CGS2
-      SG_TRY(multi_dot(S->V.p, n, j + 1, W, n, S->partial.p, S->hdev.p, false, ctx->stream), "dot");
+      // Classical Gram-Schmidt with selective re-orthogonalisation (the
+      // DGKS criterion): the projection h = V^T w comes with |w|^2 (W is
+      // V_{j+1}, so one more product in the same pass), and the second pass
+      // runs only if the first removed more than 1 - 1/sqrt(2) of |w|, i.e.
+      // if |w'|^2 < |w|^2 / 2.  The decision is taken on the device: the
+      // second pass's kernels read the two norms and return at once when it
+      // is not needed, so no host round trip is added.
+      //   hdev[0..j] = h, hdev[j+1] = |w|^2; ydev[j+1] = |w'|^2;
+      //   second pass: ydev[0..j] = V^T w', ydev[m+1] = |w''|^2
+      double* wn0 = S->hdev.p + j + 1;
+      double* wn1 = S->ydev.p + j + 1;
+      double* wn2 = S->ydev.p + m + 1;
+      SG_TRY(multi_dot(S->V.p, n, j + 2, W, n, S->partial.p, S->hdev.p, false, ctx->stream), "dot");
       SG_TRY(multi_axpy(S->V.p, n, j + 1, S->hdev.p, W, n, ctx->stream), "axpy");
-      SG_TRY(multi_dot(S->V.p, n, j + 1, W, n, S->partial.p, S->ydev.p, false, ctx->stream), "dot");
-      SG_TRY(multi_axpy(S->V.p, n, j + 1, S->ydev.p, W, n, ctx->stream), "axpy");
-      SG_TRY(multi_dot(W, 0, 1, W, n, S->partial.p, S->ydev.p + j + 1, false, ctx->stream), "dot");
-      std::vector<double> h2(j + 2);
-      SG_TRY(hipMemcpyAsync(hcol.data(), S->hdev.p, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
-      SG_TRY(hipMemcpyAsync(h2.data(), S->ydev.p, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+      SG_TRY(multi_dot(W, 0, 1, W, n, S->partial.p, wn1, false, ctx->stream), "dot");
+      SG_TRY(multi_dot(S->V.p, n, j + 1, W, n, S->partial.p, S->ydev.p, false, ctx->stream, wn1, wn0), "dot");
+      SG_TRY(multi_axpy(S->V.p, n, j + 1, S->ydev.p, W, n, ctx->stream, wn1, wn0), "axpy");
+      SG_TRY(multi_dot(W, 0, 1, W, n, S->partial.p, wn2, false, ctx->stream, wn1, wn0), "dot");
+      std::vector<double> h1(j + 2), h2(m + 2);
+      SG_TRY(hipMemcpyAsync(h1.data(), S->hdev.p, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+      SG_TRY(hipMemcpyAsync(h2.data(), S->ydev.p, (m + 2) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
       SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
-      for (int i = 0; i <= j; ++i) hcol[i] += h2[i];
-      const double hn = std::sqrt(std::max(0.0, h2[j + 1]));
+      const bool reorth = h2[j + 1] < 0.5 * h1[j + 1];  // the device's test on the same doubles
+      for (int i = 0; i <= j; ++i) hcol[i] = h1[i] + (reorth ? h2[i] : 0.0);
+      const double hn = std::sqrt(std::max(0.0, reorth ? h2[m + 1] : h2[j + 1]));
+      S->reorth_count += reorth ? 1 : 0;
       if (!std::isfinite(hn)) return sg_fail(ctx, SG_ERR_NAN, "non-finite Krylov vector");
       hcol[j + 1] = hn;
       if (hn > 0.0) SG_TRY(axpby(0.0, W, 1.0 / hn, W, n, ctx->stream), "scale");
@@ -449,7 +464,8 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     SG_TRY(launch_dt(ctx->g, ctx->ph.mu_inf, nv, q, ctx->dsmin.p, cfl, S->dtv.p, ctx->stream), "dt");
     SG_TRY(invert_dt(S->dtv.p, S->idt.p, ctx->geo.nic, ctx->geo.njc, ctx->g.ldi, ctx->stream), "invert");
     SG_TRY(launch_residual(ctx->g, ctx->ph, ctx->bc, ctx->sa, q, S->rhs.p, ctx->partial.p,
-                           reinterpret_cast<unsigned long long*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo),
+                           reinterpret_cast<unsigned long long*>(ctx->counter.p), ctx->l2.p, ctx->stream, ctx->clo, 0,
+                           &ctx->norm_parity),
            "residual");
     SG_TRY(hipMemcpyAsync(l2, ctx->l2.p, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
     SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
@@ -575,6 +591,23 @@ int sg_ls_precond_apply(sg_ctx* ctx, const double* v_aos, double* z_aos, int tra
   S->transpose = false;
   if (rc) return rc;
   return download(ctx, S->x.p, z_aos);
+}
+
+int sg_ls_precond_apply_dev_async(sg_ctx* ctx, const double* v_dev, double* z_dev, int transpose) {
+  if (!ctx || !v_dev || !z_dev) return SG_ERR_ARG;
+  if (!ctx->solver || !ctx->solver->have_lhs)
+    return sg_fail(ctx, SG_ERR_STATE, "set_lhs (after the last preallocate / set_pc / solve / adjoint call) must "
+                                      "precede precond_apply");
+  SolverState* S = ctx->solver;
+  if (S->fact_precond != S->precond || (S->precond >= 6 && (S->fact_width != S->pc_width || !S->cilu.p)))
+    return sg_fail(ctx, SG_ERR_STATE, "preconditioner factors do not match the current settings");
+  const size_t bytes = S->n * sizeof(double);
+  const char *a = (const char*)v_dev, *b = (const char*)z_dev;
+  if (a < b + bytes && b < a + bytes) return sg_fail(ctx, SG_ERR_ARG, "v_dev and z_dev overlap");
+  S->transpose = transpose != 0;
+  const int rc = precond_apply(ctx, v_dev, z_dev);
+  S->transpose = false;
+  return rc;
 }
 
 

@@ -21,8 +21,10 @@ from . import cases, lib
 # Newton-Krylov settings of the SA solves: pseudo-transient continuation with
 # switched evolution relaxation on the flux-balance norm; GMRES(30)
 # preconditioned by the chunked line ILU(0) of the first-order Jacobian
-# (lhs_order 1), which stays effective up to CFL 1e10 on 1e-6 wall spacing.
-NEWTON = dict(cfl=5.0, ser=1, ser_exponent=1.5, cfl_max=1e10, gmres_rtol=1e-4, precond=6, lhs_order=1,
+# (lhs_order 1) on the 3x3-box pattern (precond 7: that Jacobian has no +-2
+# blocks, so only fill is dropped), which stays effective up to CFL 1e10 on
+# 1e-6 wall spacing.
+NEWTON = dict(cfl=5.0, ser=1, ser_exponent=1.5, cfl_max=1e10, gmres_rtol=1e-4, precond=7, lhs_order=1,
               pc_width=32, pc_cfl_max=0.0, gmres_restart=30, gmres_maxit=150, iteration_max=200, norm_weighted=1,
               tolerance=1e-12)
 # warm starts (a converged state of a nearby problem): the initial residual is
@@ -34,7 +36,7 @@ RESOLVE = dict(WARM, iteration_max=150)
 # (measured ~1.5e-11 on the 65x49 plate against 2e-12 for plain SA)
 MLP_TOL = 1e-10
 MLP_STALL = 8
-ADJOINT = dict(restart=60, maxit=3000, rtol=1e-8, precond=6, lhs_order=1, pc_cfl=1e4)
+ADJOINT = dict(restart=60, maxit=3000, rtol=1e-8, precond=7, lhs_order=1, pc_cfl=1e4)
 
 
 def truth_coef(seed: int = 0, amp: float = 0.3) -> np.ndarray:
@@ -170,6 +172,9 @@ def stage_invert(args):
         rep = r["inv_report"]
         summary = dict(case=k, height=h, width=w, sa_s=r["sa_s"], true_s=r["true_s"], inv_s=r["inv_s"],
                        inv_iterations=rep["iterations"], J0=rep["J0"], J=rep["J"],
+                       evaluations=rep["evaluations"], failed_solves=rep["failed_solves"],
+                       adjoint_unconverged=rep["adjoint_unconverged"], adjoint_relres_max=rep["adjoint_relres_max"],
+                       adjoint_iterations=rep["adjoint_iterations"], steepest_restarts=rep["steepest_restarts"],
                        cf_err_sa=cf_error(r["cf_sa"], r["cf_true"]), cf_err_inv=cf_error(r["cf_inv"], r["cf_true"]),
                        sa_converged=bool(r["sa_report"]["converged"]),
                        true_converged=bool(r["true_report"]["converged"]))

@@ -6,8 +6,9 @@ as the plain residual (1e-12 / 1e-10).  MLP closure (BF16 matrix cores, FP32
 accumulation): beta against the oracle's FP64 network with the same BF16
 rounding points (median 1e-6, max 2e-3: rare rounding flips), the residual
 exactly equal to the field-beta residual at the beta the kernel produced, and
-J.v whose closure contribution matches the oracle's dual-number one to BF16
-accuracy while everything else matches to 1e-10."""
+J.v whose closure contribution matches the oracle's dual-number one (max
+1e-4 of its range, median 1e-8, p99 1e-5: the derivative of the network at
+the same BF16 rounding points) while everything else matches to 1e-10."""
 import numpy as np
 import pytest
 
@@ -137,7 +138,9 @@ def test_mlp_closure_jv_vs_oracle_dual(k):
         np.abs(contrib_o).max(), rel(contrib[..., 4], contrib_o[..., 4]), np.median(e), np.quantile(e, 0.99),
         rel(jv[..., :4], jvo[..., :4])))
     assert np.abs(contrib_o[..., 4]).max() > 1e-4 * np.abs(jvo[..., 4]).max()
-    assert rel(contrib[..., 4], contrib_o[..., 4]) <= 2e-2
+    # measured: max 1.7e-6, median 8.5e-10 (BF16 rounding flips of single cells)
+    assert rel(contrib[..., 4], contrib_o[..., 4]) <= 1e-4
+    assert np.median(e) <= 1e-8 and np.quantile(e, 0.99) <= 1e-5
     assert rel(jv[..., :4], jvo[..., :4]) <= 1e-10
     assert np.abs(contrib[..., :4]).max() <= 1e-10 * np.abs(jvo[..., :4]).max()
 

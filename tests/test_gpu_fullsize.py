@@ -123,6 +123,7 @@ def test_fullsize_mlp_closure_residual_and_jv():
         print("1409x641 closure part of J.v: median %.2e p99.9 %.2e max %.2e; rest %.2e" % (
             np.median(e), np.quantile(e, 0.999), e.max(), rel(jv[..., :4], jvo[..., :4])), flush=True)
         assert rel(jv[..., :4], jvo[..., :4]) <= 1e-10
-        assert np.median(e) <= 1e-6 and e.max() <= 2e-2
+        # measured: median 9e-18, p99.9 1e-9, max 1.1e-6
+        assert np.median(e) <= 1e-8 and np.quantile(e, 0.999) <= 1e-6 and e.max() <= 1e-4
     finally:
         of.set_closure(0)

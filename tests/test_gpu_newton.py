@@ -179,3 +179,36 @@ def test_first_order_preconditioner_blocks():
           float(np.abs(z2 - z1).max() / np.abs(z1).max()))
     assert err <= 1e-9
     assert np.abs(z2 - z1).max() > 1e-3 * np.abs(z1).max()
+
+
+@pytest.mark.parametrize("precond", [6, 7])
+def test_precond_apply_dev_async_equals_host_entry(precond):
+    """sg_ls_precond_apply_dev_async (device SoA vectors, the context's
+    stream) gives the host entry point's z bit for bit, M^-1 and M^-T, and
+    refuses overlapping v and z."""
+    import ctypes as C
+    torch = pytest.importorskip("torch")
+    so = lib.load()
+    case = cases.sa_bump(41, 17, dy1=1e-3)
+    q = case.perturbed_state(seed=6, amp=0.01)
+    eq = lib.Equation(case)
+    dt = eq.calc_dt(q, 50.0)
+    eq.ls_preallocate(restart=10, precond=precond, maxit=10, rtol=1e-6)
+    eq.ls_set_pc(1, 1.0, 16)
+    eq.ls_set_lhs(q, dt)
+    v = np.random.default_rng(5).standard_normal(q.shape)
+    dev = torch.device("cuda:0")
+    aos = torch.from_numpy(np.ascontiguousarray(v).reshape(-1)).to(dev)
+    vd = torch.zeros(eq.nvar * eq.plane, dtype=torch.float64, device=dev)
+    eq.aos_to_soa_dev(aos.data_ptr(), vd.data_ptr())
+    for transpose in (0, 1):
+        zh = eq.ls_precond_apply(v, transpose=bool(transpose))
+        zd = torch.full_like(vd, -1.0)
+        rc = so.sg_ls_precond_apply_dev_async(eq.h, C.c_void_p(vd.data_ptr()), C.c_void_p(zd.data_ptr()), transpose)
+        assert rc == 0, so.sg_last_error(eq.h)
+        za = torch.empty_like(aos)
+        eq.soa_to_aos_dev(zd.data_ptr(), za.data_ptr())
+        eq.synchronize()
+        np.testing.assert_array_equal(za.cpu().numpy().reshape(q.shape), zh)
+    rc = so.sg_ls_precond_apply_dev_async(eq.h, C.c_void_p(vd.data_ptr()), C.c_void_p(vd.data_ptr() + 8), 0)
+    assert lib.ERRORS[rc] == "SG_ERR_ARG"
