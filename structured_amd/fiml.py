@@ -269,25 +269,43 @@ def stage_reference(args):
         eq.set_beta(beta0)
         q0, _, _ = timed_solve(eq, q_sa, **WARM)
         print("resumed from %s after %d iterations" % (args.resume, done), flush=True)
-    t0 = time.perf_counter()
-    q_inv, beta_inv, irep, jh = eq.invert(q0, cf_true, beta=beta0, iterations=args.iterations,
-                                          newton=dict(RESOLVE, verbose=1), adjoint=ADJOINT)
-    t_inv = time.perf_counter() - t0
-    _, _, cf_inv, _ = eq.wall(q_inv)
-    eq.close()
+    # the inversion runs in chunks of --chunk iterations, saving after each,
+    # so that a run cut short keeps what it did (the L-BFGS memory restarts
+    # at every chunk, as it does on --resume)
     os.makedirs(args.out, exist_ok=True)
-    total = done + irep["iterations"]
-    np.savez_compressed(os.path.join(args.out, "reference_%s.npz" % args.test_grid), beta_inv=beta_inv,
-                        cf_inv=cf_inv, cf_true=cf_true, cf_sa=cf_sa, J_hist=jh, iterations_done=total,
-                        inversion_s=wall_before + t_inv)
-    out = dict(test_grid=args.test_grid, sa_s=t_sa, sa_newton=r_sa["iterations"], truth_s=t_true,
-               iterations_this_run=irep["iterations"], iterations_done=total, inversion_s_this_run=t_inv,
-               inversion_s_total=wall_before + t_inv, J0=irep["J0"], J=irep["J"], evaluations=irep["evaluations"],
-               failed_solves=irep["failed_solves"], cf_err_sa_vs_truth=cf_error(cf_sa, cf_true),
-               cf_err_inv_vs_truth=cf_error(cf_inv, cf_true))
-    with open(os.path.join(args.out, "reference_%s.json" % args.test_grid), "w") as f:
-        json.dump(out, f, indent=1)
-    print("reference", json.dumps(out), flush=True)
+    chunk = args.chunk if args.chunk > 0 else args.iterations
+    q_cur, beta_cur = q0, beta0
+    t_inv, its, evals, failed, adj_unconv = 0.0, 0, 0, 0, 0
+    J0, J, jhs = None, None, []
+    while its < args.iterations:
+        n = min(chunk, args.iterations - its)
+        t0 = time.perf_counter()
+        q_cur, beta_cur, irep, jh = eq.invert(q_cur, cf_true, beta=beta_cur, iterations=n,
+                                              newton=dict(RESOLVE, verbose=1), adjoint=ADJOINT)
+        t_inv += time.perf_counter() - t0
+        J0 = irep["J0"] if J0 is None else J0
+        J = irep["J"]
+        its += irep["iterations"]
+        evals += irep["evaluations"]
+        failed += irep["failed_solves"]
+        adj_unconv += irep["adjoint_unconverged"]
+        jhs.append(np.asarray(jh)[:irep["iterations"] + 1] if not jhs else np.asarray(jh)[1:irep["iterations"] + 1])
+        _, _, cf_inv, _ = eq.wall(q_cur)
+        total = done + its
+        np.savez_compressed(os.path.join(args.out, "reference_%s.npz" % args.test_grid), beta_inv=beta_cur,
+                            cf_inv=cf_inv, cf_true=cf_true, cf_sa=cf_sa, J_hist=np.concatenate(jhs),
+                            iterations_done=total, inversion_s=wall_before + t_inv)
+        out = dict(test_grid=args.test_grid, sa_s=t_sa, sa_newton=r_sa["iterations"], truth_s=t_true,
+                   iterations_this_run=its, iterations_done=total, inversion_s_this_run=t_inv,
+                   inversion_s_total=wall_before + t_inv, J0=J0, J=J, evaluations=evals, failed_solves=failed,
+                   adjoint_unconverged=adj_unconv, chunk=chunk, cf_err_sa_vs_truth=cf_error(cf_sa, cf_true),
+                   cf_err_inv_vs_truth=cf_error(cf_inv, cf_true))
+        with open(os.path.join(args.out, "reference_%s.json" % args.test_grid), "w") as f:
+            json.dump(out, f, indent=1)
+        print("reference", json.dumps(out), flush=True)
+        if irep["iterations"] < n:  # the line search found no further descent
+            break
+    eq.close()
 
 
 def main(argv=None):
@@ -307,6 +325,7 @@ def main(argv=None):
     ap.add_argument("--out", default="fiml_data")
     ap.add_argument("--net-out", default="fiml_data/closure_net.npz")
     ap.add_argument("--resume", default="", help="reference stage: saved reference_*.npz to continue from")
+    ap.add_argument("--chunk", type=int, default=10, help="reference stage: iterations per saved chunk (0: all)")
     args = ap.parse_args(argv)
     {"invert": stage_invert, "test": stage_test, "train": stage_train, "reference": stage_reference}[args.stage](args)
 

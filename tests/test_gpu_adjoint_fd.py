@@ -6,7 +6,8 @@ the one-sided differences.
 F(beta) = sum_i ds_i (Cf_i(q(beta)) - Cf*_i)^2 with R(q(beta), beta) = 0;
 dF/dbeta from one adjoint solve J^T psi = dF/dq and dF/dbeta = -psi^T dR/dbeta.
 Each finite difference re-converges the primal from the base state
-(flux-balance norm below 1e-15) at beta_c +- eps."""
+(flux-balance norm below 1e-15) at beta_c +- eps.  The 10 cells run as two
+tests of five and a summary test."""
 import numpy as np
 import pytest
 
@@ -26,7 +27,12 @@ def beta_field(case, seed, nblob=8, amp=0.2):
     return cases.blob_beta(case, seed, nblob, amp)
 
 
-def test_adjoint_gradient_matches_central_differences():
+GROUPS = 2        # the 10 cells run as 2 tests of 5, so that each test ends within a minute or so
+_RESULTS = {}
+
+
+@pytest.fixture(scope="module")
+def base():
     case = cases.sa_bump(49, 25, dy1=1e-4)
     eq = lib.Equation(case)
     q1, rep, _ = eq.solve(case.freestream_state(), **NEWTON)
@@ -43,8 +49,13 @@ def test_adjoint_gradient_matches_central_differences():
     # 10 cells in the region the wall Cf is sensitive to (near the wall): the
     # first ten of a seeded draw of 16
     cells = [(int(i), int(j)) for i, j in zip(rng.integers(2, case.nic - 2, 16), rng.integers(0, case.njc // 3, 16))][:10]
-    worst = 0.0
-    rels, kinks = [], []
+    yield dict(case=case, eq=eq, q=q, beta=beta, g=g, F=F, cells=cells, cf_target=cf_target)
+    eq.close()
+
+
+@pytest.mark.parametrize("group", range(GROUPS))
+def test_adjoint_gradient_matches_central_differences(base, group):
+    eq, beta, g, F, q = base["eq"], base["beta"], base["g"], base["F"], base["q"]
 
     def F_at(i, j, db):
         b = beta.copy()
@@ -52,10 +63,11 @@ def test_adjoint_gradient_matches_central_differences():
         eq.set_beta(b)
         qp, r2, _ = eq.solve(q, **RESOLVE)  # to the roundoff floor
         print("  re-solve", (i, j), db, "newton", r2["iterations"], flush=True)
-        return eq.objective_cf(qp, cf_target, grad=False)[0]
+        return eq.objective_cf(qp, base["cf_target"], grad=False)[0]
 
     gmax = np.abs(g).max()
-    for (i, j) in cells:
+    n = len(base["cells"]) // GROUPS
+    for (i, j) in base["cells"][group * n:(group + 1) * n]:
         e = 1e-2
         Fp1, Fm1, Fp2, Fm2 = F_at(i, j, e), F_at(i, j, -e), F_at(i, j, 2 * e), F_at(i, j, -2 * e)
         # Richardson-extrapolated central difference, O(eps^4)
@@ -63,8 +75,7 @@ def test_adjoint_gradient_matches_central_differences():
         fwd, bwd = (Fp1 - F) / e, (F - Fm1) / e
         bracketed = min(fwd, bwd) <= g[i, j] <= max(fwd, bwd)
         rel = abs(g[i, j] - fd) / max(abs(fd), 1e-2 * gmax)
-        worst = max(worst, rel)
-        rels.append(rel)
+        _RESULTS[(i, j)] = rel
         print("cell", (i, j), "adjoint %.10e fd %.10e rel %.2e | one-sided %.8e %.8e bracketed %s" % (
             g[i, j], fd, rel, fwd, bwd, bracketed), flush=True)
         # the bar is 1e-6; a cell whose +-eps perturbation moves a face across
@@ -72,9 +83,16 @@ def test_adjoint_gradient_matches_central_differences():
         # u_n ~ 0) has different left and right derivatives: the adjoint value
         # (one branch) must then lie between the one-sided differences
         if rel > 1e-6:
-            kinks.append((i, j))
             assert bracketed, ((i, j), rel)
     eq.set_beta(beta)
-    print("median rel %.2e max rel %.2e; cells on a kink (bracketed): %s" % (np.median(rels), worst, kinks))
+
+
+def test_adjoint_gradient_central_differences_summary(base):
+    """Over the 10 cells: median relative difference <= 1e-7 and at most 3
+    cells on a kink (each bracketed, checked above)."""
+    assert len(_RESULTS) == len(base["cells"]), "the cell groups must run first"
+    rels = list(_RESULTS.values())
+    kinks = [c for c, r in _RESULTS.items() if r > 1e-6]
+    print("median rel %.2e max rel %.2e; cells on a kink (bracketed): %s" % (np.median(rels), max(rels), kinks))
     assert np.median(rels) <= 1e-7
     assert len(kinks) <= 3

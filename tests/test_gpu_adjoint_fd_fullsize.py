@@ -19,7 +19,8 @@ two nearby Cf distributions, so its value is resolved only to about 1e-9
 relative (printed below as the noise floor: the same beta re-converged from
 another state); one cell of this grid moves F by about 1e-5 relative, too
 little for a finite difference to resolve 1e-6, while a bump of ~30 cells
-moves it by enough."""
+moves it by enough.  The 16 directions run as four tests of four and a
+summary test."""
 import numpy as np
 import pytest
 
@@ -28,9 +29,12 @@ from structured_amd import cases, fiml, lib
 pytestmark = pytest.mark.gpu
 
 RESOLVE = dict(fiml.WARM, tolerance=1e-17, stall_window=4, iteration_max=40)
+GROUPS = 4        # the 16 directions run as 4 tests of 4, so that each test ends within a minute or so
+_RESULTS = {}
 
 
-def test_adjoint_gradient_fd_705x321_16_cells():
+@pytest.fixture(scope="module")
+def base():
     case = cases.sa_bump(705, 321)
     eq = lib.Equation(case)
     q1, r1, _ = eq.solve(case.freestream_state(), **fiml.NEWTON)
@@ -54,15 +58,22 @@ def test_adjoint_gradient_fd_705x321_16_cells():
     sens = np.argwhere(np.abs(g) >= 1e-2 * gmax)
     rng = np.random.default_rng(16)
     cells = [tuple(int(v) for v in sens[k]) for k in rng.choice(len(sens), 16, replace=False)]
+    yield dict(case=case, eq=eq, q=q, beta=beta, g=g, F0=F0, noise=noise, cells=cells, cf_target=cf_target)
+    eq.close()
+
+
+@pytest.mark.parametrize("group", range(GROUPS))
+def test_adjoint_gradient_fd_705x321_directions(base, group):
+    eq, case, beta, g, F0, q = base["eq"], base["case"], base["beta"], base["g"], base["F0"], base["q"]
     ii, jj = np.meshgrid(np.arange(case.nic), np.arange(case.njc), indexing="ij")
 
     def F_at(d, e):
         eq.set_beta(beta + e * d)
         qp, rp, _ = eq.solve(q, **RESOLVE)
-        return eq.objective_cf(qp, cf_target, grad=False)[0], rp
+        return eq.objective_cf(qp, base["cf_target"], grad=False)[0], rp
 
-    rels, kinks = [], []
-    for (i, j) in cells:
+    n = len(base["cells"]) // GROUPS
+    for (i, j) in base["cells"][group * n:(group + 1) * n]:
         d = np.exp(-((ii - i) ** 2 + (jj - j) ** 2) / 9.0)
         d[d < 1e-6] = 0.0
         gd = float(np.sum(g * d))
@@ -76,14 +87,21 @@ def test_adjoint_gradient_fd_705x321_16_cells():
         rel = abs(gd - fd) / abs(fd)
         fwd, bwd = (Fp1 - F0) / e, (F0 - Fm1) / e
         bracketed = min(fwd, bwd) <= gd <= max(fwd, bwd)
-        rels.append(rel)
+        _RESULTS[(i, j)] = (rel, bracketed)
         print("cell", (i, j), "g.d %.12e fd %.12e rel %.2e | one-sided %.9e %.9e bracketed %s | dF/F %.1e "
               "fd noise bound %.1e | resolve %d" % (gd, fd, rel, fwd, bwd, bracketed, abs(Fp1 - F0) / F0,
-                                                  1.5 * noise / e / abs(fd), rp["iterations"]), flush=True)
+                                                  1.5 * base["noise"] / e / abs(fd), rp["iterations"]), flush=True)
         if rel > 1e-6:
-            kinks.append((i, j))
             assert bracketed, ((i, j), rel)
     eq.set_beta(beta)
+
+
+def test_adjoint_gradient_fd_705x321_summary(base):
+    """Over the 16 directions: median relative difference <= 1e-6 and at
+    most 4 directions across a kink (each bracketed, checked above)."""
+    assert len(_RESULTS) == len(base["cells"]), "the direction groups must run first"
+    rels = [r for r, _ in _RESULTS.values()]
+    kinks = [c for c, (r, _) in _RESULTS.items() if r > 1e-6]
     print("median rel %.2e max rel %.2e; directions across a kink (bracketed): %s" % (
         np.median(rels), max(rels), kinks))
     assert np.median(rels) <= 1e-6
