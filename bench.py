@@ -132,7 +132,7 @@ def adjoint_leg(grid, cpu=True):
     t0 = time.perf_counter()
     q, r3, _ = eq.solve(q1, **fiml.WARM)
     t_warm = time.perf_counter() - t0
-    adj = dict(fiml.ADJOINT, restart=120, maxit=4000)
+    adj = dict(fiml.ADJOINT)
     t0 = time.perf_counter()
     g, F, rep = eq.adjoint_gradient(q, cf_target, **adj)
     t_adj = time.perf_counter() - t0

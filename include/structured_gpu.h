@@ -138,7 +138,8 @@ typedef struct sg_adjoint_opts {
 } sg_adjoint_opts;
 typedef struct sg_adjoint_report {
   int iterations;
-  int reserved;
+  int reorthogonalised;  /* Arnoldi steps that took the second Gram-Schmidt
+                            pass (selective re-orthogonalisation)           */
   double relres;
   double wall_s;
 } sg_adjoint_report;

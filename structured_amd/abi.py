@@ -106,7 +106,8 @@ class AdjointOpts(C.Structure):
 
 
 class AdjointReport(C.Structure):
-    _fields_ = [("iterations", C.c_int), ("reserved", C.c_int), ("relres", C.c_double), ("wall_s", C.c_double)]
+    _fields_ = [("iterations", C.c_int), ("reorthogonalised", C.c_int), ("relres", C.c_double),
+                ("wall_s", C.c_double)]
 
 
 class InversionOpts(C.Structure):

@@ -712,12 +712,14 @@ int sg_adjoint_solve(sg_ctx* ctx, const double* q_aos, const double* rhs_aos, do
   S->transpose = true;
   int it = 0;
   double rr = 0.0;
+  const long reorth0 = S->reorth_count;
   rc = fgmres(ctx, o->rtol, o->maxit, &it, &rr);
   S->transpose = false;
   if (rc) return rc;
   if ((rc = download(ctx, S->x.p, psi_aos))) return rc;
   if (rep) {
     rep->iterations = it;
+    rep->reorthogonalised = (int)(S->reorth_count - reorth0);
     rep->relres = rr;
     rep->wall_s = t.s();
   }

@@ -334,7 +334,8 @@ class Equation:
         rc = load().sg_adjoint_solve(self.h, _p(q), _p(rhs), _p(psi), C.byref(o), C.byref(rep))
         if rc not in (0, -5):
             self._check(rc)
-        return psi, {"iterations": rep.iterations, "relres": rep.relres, "wall_s": rep.wall_s, "converged": rc == 0}
+        return psi, {"iterations": rep.iterations, "relres": rep.relres, "wall_s": rep.wall_s,
+                     "reorthogonalised": rep.reorthogonalised, "converged": rc == 0}
 
     def objective_cf(self, q, cf_target, grad=True):
         q = _f64(q, self.shape)
@@ -356,7 +357,7 @@ class Equation:
         if rc not in (0, -5):
             self._check(rc)
         return gb, F.value, {"iterations": rep.iterations, "relres": rep.relres, "wall_s": rep.wall_s,
-                             "converged": rc == 0}
+                             "reorthogonalised": rep.reorthogonalised, "converged": rc == 0}
 
     def invert(self, q, cf_target, beta=None, iterations=50, memory=10, reg=0.0, step0=0.2, beta_min=0.0,
                beta_max=5.0, max_backtracks=6, newton=None, adjoint=None):
