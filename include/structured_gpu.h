@@ -134,6 +134,8 @@ typedef struct sg_adjoint_opts {
                       6 chunked line ILU(0) on the 13-point stencil, 7 the
                       same on the 3x3-box pattern                           */
   int lhs_order;   /* 1: the preconditioner factors the first-order Jacobian */
+  int pc_width;    /* chunk width of precond 6 / 7 (8, 16 or 32); 0 keeps the
+                      context's current width                               */
   double pc_cfl;   /* the preconditioner factors I/dt - J at this CFL        */
 } sg_adjoint_opts;
 typedef struct sg_adjoint_report {

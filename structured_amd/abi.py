@@ -102,7 +102,7 @@ def explicit_opts(scheme="forward_euler", iteration_max=1, cfl=1.0, cfl_ramp=Fal
 
 class AdjointOpts(C.Structure):
     _fields_ = [("restart", C.c_int), ("maxit", C.c_int), ("rtol", C.c_double), ("precond", C.c_int),
-                ("lhs_order", C.c_int), ("pc_cfl", C.c_double)]
+                ("lhs_order", C.c_int), ("pc_width", C.c_int), ("pc_cfl", C.c_double)]
 
 
 class AdjointReport(C.Structure):

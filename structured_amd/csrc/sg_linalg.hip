@@ -11,46 +11,49 @@
 namespace sg {
 
 constexpr int DOT_BLOCK = 256;
-constexpr int DOT_GRID = 2048;  // partial sums per dot product (8 blocks per CU)
 
 // Gate of the conditional kernels: with ga and gb (device) the kernel runs
 // only when *ga < *gb / 2, else it returns at once (nothing written).
 __device__ __forceinline__ bool gate_open(const double* ga, const double* gb) { return !ga || *ga < 0.5 * *gb; }
 
-// out_partial[b*k + i] = sum over this block's slice of V_i . w, for groups
-// of DOT_KC vectors at a time (accumulators in registers), each summed over
-// the block in a fixed order (DPP wavefront sums, then the four waves).
+// out_partial[b*k + i] = sum over slice b of V_i . w.  The grid is
+// DOT_SLICES x (groups of DOT_KC vectors): block (b, g) sums vectors
+// g*DOT_KC .. g*DOT_KC + DOT_KC - 1 over the elements of slice b (the
+// elements b*DOT_BLOCK + t + m*DOT_SLICES*DOT_BLOCK), with its accumulators in
+// registers, then over the block in a fixed order (DPP wavefront sums, then
+// the four waves).  All groups run at once, so a launch keeps every vector's
+// loads in flight together; each group reads w once more, from L2 / MALL
+// after the first.
 constexpr int DOT_KC = 8;
+constexpr int DOT_SLICES = 512;  // partial sums per dot product
 __global__ __launch_bounds__(DOT_BLOCK) void multi_dot_kernel(const double* V, size_t ldv, int k, const double* w,
                                                               size_t n, double* partial, const double* ga,
                                                               const double* gb) {
   if (!gate_open(ga, gb)) return;
   __shared__ double red[DOT_BLOCK / 64][DOT_KC];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int i0 = 0; i0 < k; i0 += DOT_KC) {
-    const int kk = min(DOT_KC, k - i0);
-    double acc[DOT_KC];
+  const int i0 = (int)blockIdx.y * DOT_KC, kk = min(DOT_KC, k - i0);
+  double acc[DOT_KC];
 #pragma unroll
-    for (int i = 0; i < DOT_KC; ++i) acc[i] = 0.0;
-    for (size_t e = (size_t)blockIdx.x * DOT_BLOCK + threadIdx.x; e < n; e += (size_t)gridDim.x * DOT_BLOCK) {
-      const double we = w[e];
+  for (int i = 0; i < DOT_KC; ++i) acc[i] = 0.0;
+#pragma unroll 2
+  for (size_t e = (size_t)blockIdx.x * DOT_BLOCK + threadIdx.x; e < n; e += (size_t)DOT_SLICES * DOT_BLOCK) {
+    const double we = w[e];
 #pragma unroll
-      for (int i = 0; i < DOT_KC; ++i)
-        if (i < kk) acc[i] += V[(size_t)(i0 + i) * ldv + e] * we;
-    }
+    for (int i = 0; i < DOT_KC; ++i)
+      if (i < kk) acc[i] += V[(size_t)(i0 + i) * ldv + e] * we;
+  }
 #pragma unroll
-    for (int i = 0; i < DOT_KC; ++i) acc[i] = wave_sum_lane63(acc[i]);
-    if (lane == 63) {
+  for (int i = 0; i < DOT_KC; ++i) acc[i] = wave_sum_lane63(acc[i]);
+  if (lane == 63) {
 #pragma unroll
-      for (int i = 0; i < DOT_KC; ++i) red[wv][i] = acc[i];
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < kk) {
-      double s = 0.0;
-      for (int q = 0; q < DOT_BLOCK / 64; ++q) s += red[q][threadIdx.x];
-      partial[(size_t)blockIdx.x * k + i0 + threadIdx.x] = s;
-    }
-    __syncthreads();
+    for (int i = 0; i < DOT_KC; ++i) red[wv][i] = acc[i];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < kk) {
+    double t = 0.0;
+    for (int q = 0; q < DOT_BLOCK / 64; ++q) t += red[q][threadIdx.x];
+    partial[(size_t)blockIdx.x * k + i0 + threadIdx.x] = t;
   }
 }
 
@@ -73,13 +76,14 @@ __global__ __launch_bounds__(256) void dot_finalize_kernel(const double* partial
 
 hipError_t multi_dot(const double* V, size_t ldv, int k, const double* w, size_t n, double* partial, double* out,
                      bool accumulate, hipStream_t s, const double* ga, const double* gb) {
-  hipLaunchKernelGGL(multi_dot_kernel, dim3(DOT_GRID), dim3(DOT_BLOCK), 0, s, V, ldv, k, w, n, partial, ga, gb);
-  hipLaunchKernelGGL(dot_finalize_kernel, dim3(k), dim3(256), 0, s, partial, DOT_GRID, k, out, accumulate ? 1 : 0, ga,
-                     gb);
+  const dim3 grid(DOT_SLICES, (unsigned)((k + DOT_KC - 1) / DOT_KC));
+  hipLaunchKernelGGL(multi_dot_kernel, grid, dim3(DOT_BLOCK), 0, s, V, ldv, k, w, n, partial, ga, gb);
+  hipLaunchKernelGGL(dot_finalize_kernel, dim3(k), dim3(256), 0, s, partial, DOT_SLICES, k, out, accumulate ? 1 : 0,
+                     ga, gb);
   return hipGetLastError();
 }
 
-size_t multi_dot_partial_size(int kmax) { return (size_t)DOT_GRID * kmax; }
+size_t multi_dot_partial_size(int kmax) { return (size_t)DOT_SLICES * kmax; }
 
 // w -= sum_i c[i] V_i   (c in device memory)
 __global__ void multi_axpy_kernel(const double* V, size_t ldv, int k, const double* c, double* w, size_t n,

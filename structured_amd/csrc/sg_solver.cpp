@@ -687,9 +687,12 @@ int sg_adjoint_solve(sg_ctx* ctx, const double* q_aos, const double* rhs_aos, do
   if (o->precond != 0 && o->precond != 4 && o->precond != 5 && o->precond != 6 && o->precond != 7)
     return sg_fail(ctx, SG_ERR_ARG, "adjoint precond must be 0, 4, 5, 6 or 7");
   if (o->lhs_order < 0 || o->lhs_order > 2) return sg_fail(ctx, SG_ERR_ARG, "adjoint lhs_order must be 0, 1 or 2");
+  if (o->pc_width != 0 && !cilu_width_ok(o->pc_width))
+    return sg_fail(ctx, SG_ERR_ARG, "adjoint pc_width must be 0, 8, 16 or 32");
   int rc;
   if ((rc = sg_ls_preallocate(ctx, o->restart, o->precond, o->maxit, o->rtol))) return rc;
   SolverState* S = ctx->solver;
+  if (o->pc_width) S->pc_width = o->pc_width;
   Timer t;
   if ((rc = upload(ctx, q_aos, ctx->q.p))) return rc;
   if ((rc = adjoint_prepare(ctx, ctx->q.p))) return rc;

@@ -324,12 +324,13 @@ class Equation:
         return z
 
     # ---- adjoint
-    def adjoint_solve(self, q, rhs, restart=40, maxit=400, rtol=1e-10, precond=4, pc_cfl=1e3, lhs_order=0):
+    def adjoint_solve(self, q, rhs, restart=40, maxit=400, rtol=1e-10, precond=4, pc_cfl=1e3, lhs_order=0,
+                       pc_width=0):
         q = _f64(q, self.shape)
         rhs = _f64(rhs, self.shape)
         psi = np.empty(self.shape)
         o = abi.AdjointOpts(restart=restart, maxit=maxit, rtol=rtol, precond=precond, lhs_order=lhs_order,
-                            pc_cfl=pc_cfl)
+                            pc_width=pc_width, pc_cfl=pc_cfl)
         rep = abi.AdjointReport()
         rc = load().sg_adjoint_solve(self.h, _p(q), _p(rhs), _p(psi), C.byref(o), C.byref(rep))
         if rc not in (0, -5):
@@ -345,13 +346,14 @@ class Equation:
         self._check(load().sg_objective_cf(self.h, _p(q), _p(t), C.byref(F), _p(g)))
         return F.value, g
 
-    def adjoint_gradient(self, q, cf_target, restart=40, maxit=400, rtol=1e-10, precond=4, pc_cfl=1e3, lhs_order=0):
+    def adjoint_gradient(self, q, cf_target, restart=40, maxit=400, rtol=1e-10, precond=4, pc_cfl=1e3, lhs_order=0,
+                          pc_width=0):
         q = _f64(q, self.shape)
         t = _f64(cf_target)
         gb = np.empty((self.nic, self.njc))
         F = C.c_double()
         o = abi.AdjointOpts(restart=restart, maxit=maxit, rtol=rtol, precond=precond, lhs_order=lhs_order,
-                            pc_cfl=pc_cfl)
+                            pc_width=pc_width, pc_cfl=pc_cfl)
         rep = abi.AdjointReport()
         rc = load().sg_adjoint_gradient(self.h, _p(q), _p(t), _p(gb), C.byref(F), C.byref(o), C.byref(rep))
         if rc not in (0, -5):
