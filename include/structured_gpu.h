@@ -383,7 +383,9 @@ typedef struct sg_inversion_report {
 } sg_inversion_report;
 /* q_aos: converged state at the initial beta (in), state at the final beta
  * (out); beta [nic][njc] in/out; J_hist (iterations + 1 entries, may be NULL)
- * receives J at every accepted iterate. */
+ * receives J at every accepted iterate.  A primal re-solve that stops on
+ * opts->newton.stall_window (at its roundoff floor, within 1000x of the
+ * tolerance) counts as converged. */
 int sg_invert(sg_ctx* ctx, double* q_aos, const double* cf_target, double* beta, const sg_inversion_opts* opts,
               sg_inversion_report* report, double* J_hist);
 

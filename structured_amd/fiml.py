@@ -30,7 +30,13 @@ NEWTON = dict(cfl=5.0, ser=1, ser_exponent=1.5, cfl_max=1e10, gmres_rtol=1e-4, p
 # warm starts (a converged state of a nearby problem): the initial residual is
 # small, so SER would keep the CFL near its start; start high instead
 WARM = dict(NEWTON, cfl=1e5)
-RESOLVE = dict(WARM, iteration_max=150)
+# re-solves inside the inversion: a change of beta late in an inversion can
+# leave ||V R|| below 1e-12, so that a re-solve to that tolerance would not
+# take a single Newton step and the objective would not see the change (the
+# 1409x641 inversion stopped so after 7 iterations); these run to the
+# roundoff floor instead (stall_window: no halving over 3 steps, within 1000x
+# of the tolerance)
+RESOLVE = dict(WARM, iteration_max=150, tolerance=1e-15, stall_window=3)
 # the BF16 closure network makes beta a piecewise-constant function of the
 # state at its rounding points, which sets a floor on the flux-balance norm
 # (measured ~1.5e-11 on the 65x49 plate against 2e-12 for plain SA)
