@@ -357,8 +357,10 @@ def cpu_baseline(case, q, budget_s):
     t_single = of.time_residual(case, q, warm=0, reps=1, threads=1)
     return {"value": cells / t / 1e6, "unit": "Mcells/s", "cores": threads, "kind": "port",
             "sample": "median of %d residual evaluations of the same %dx%d SA grid and state (after 1 warm-up), "
-                      "OpenMP over %d host threads; 1-thread: %.4f Mcells/s" % (reps, case.ni, case.nj, threads,
-                                                                                 cells / t_single / 1e6),
+                      "OpenMP over %d host threads; 1-thread: %.4f Mcells/s (the primitive, reconstruction, "
+                      "convective-flux and assembly loops are threaded; the boundary conditions, face gradients "
+                      "and viscous fluxes run serially as in the reference, so threads add little)"
+                      % (reps, case.ni, case.nj, threads, cells / t_single / 1e6),
             "single_thread_mcells_s": cells / t_single / 1e6}
 
 
