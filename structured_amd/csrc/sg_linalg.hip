@@ -85,13 +85,25 @@ hipError_t multi_dot(const double* V, size_t ldv, int k, const double* w, size_t
 
 size_t multi_dot_partial_size(int kmax) { return (size_t)DOT_SLICES * kmax; }
 
-// w -= sum_i c[i] V_i   (c in device memory)
+// w -= sum_i c[i] V_i   (c in device memory).  The vectors are taken eight
+// at a time with their loads issued together ahead of the eight updates (the
+// updates keep the order i = 0, 1, ..., so the result does not depend on the
+// grouping); a loop with one load per update waited for every load in turn.
+constexpr int AXPY_KC = 8;
 __global__ void multi_axpy_kernel(const double* V, size_t ldv, int k, const double* c, double* w, size_t n,
                                   const double* ga, const double* gb) {
   if (!gate_open(ga, gb)) return;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     double s = w[e];
-    for (int i = 0; i < k; ++i) s -= c[i] * V[(size_t)i * ldv + e];
+    int i = 0;
+    for (; i + AXPY_KC <= k; i += AXPY_KC) {
+      double v[AXPY_KC];
+#pragma unroll
+      for (int u = 0; u < AXPY_KC; ++u) v[u] = V[(size_t)(i + u) * ldv + e];
+#pragma unroll
+      for (int u = 0; u < AXPY_KC; ++u) s -= c[i + u] * v[u];
+    }
+    for (; i < k; ++i) s -= c[i] * V[(size_t)i * ldv + e];
     w[e] = s;
   }
 }
@@ -102,11 +114,19 @@ hipError_t multi_axpy(const double* V, size_t ldv, int k, const double* c, doubl
   return hipGetLastError();
 }
 
-// y = sum_i c[i] X_i  (c on the host, k small)
+// y = sum_i c[i] X_i  (c in device memory), grouped as multi_axpy
 __global__ void lincomb_kernel(const double* X, size_t ldx, int k, const double* c, double* y, size_t n, int accumulate) {
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     double s = accumulate ? y[e] : 0.0;
-    for (int i = 0; i < k; ++i) s += c[i] * X[(size_t)i * ldx + e];
+    int i = 0;
+    for (; i + AXPY_KC <= k; i += AXPY_KC) {
+      double v[AXPY_KC];
+#pragma unroll
+      for (int u = 0; u < AXPY_KC; ++u) v[u] = X[(size_t)(i + u) * ldx + e];
+#pragma unroll
+      for (int u = 0; u < AXPY_KC; ++u) s += c[i + u] * v[u];
+    }
+    for (; i < k; ++i) s += c[i] * X[(size_t)i * ldx + e];
     y[e] = s;
   }
 }
