@@ -35,6 +35,31 @@ struct DevBuf {
   }
 };
 
+// Page-locked host memory (hipHostMalloc), for the small per-step reads of
+// the Krylov loop: a copy into it is a DMA the stream orders, where a copy
+// into pageable memory goes through a staging buffer.
+struct PinnedBuf {
+  double* p = nullptr;
+  size_t n = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+  hipError_t alloc(size_t count) {
+    release();
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipHostMalloc((void**)&p, count * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) p = nullptr;
+    else n = count;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
 // Mesh::calc_metrics, src/utils/mesh.cpp:171-205, reference layouts.
 struct HostGeom {
   int ni = 0, nj = 0, nic = 0, njc = 0;

@@ -43,6 +43,7 @@ struct SolverState {
   DevBuf V, Z, blocks, fac, ilu, idt, dtv, rhs, x, tmp, jz, qop, seed, hdev, ydev, partial, gst, dbdiag, ones;
   DevBuf blocks_pc;          // preconditioner blocks when lhs_order > 0 (blocks keeps the exact J)
   DevBuf cilu;               // chunked line ILU(0) factors (precond 6)
+  PinnedBuf hpin;            // host copies of hdev and ydev for the Arnoldi step
   int pc_width = 16;         // chunk width of precond 6
   // have_lhs: the operator and preconditioner of the caller's last set_lhs are
   // in place.  Every call that changes the preconditioner settings or
@@ -86,7 +87,8 @@ int ensure(sg_ctx* ctx, int restart) {
   if (S->V.alloc((size_t)(restart + 1) * n) || S->Z.alloc((size_t)restart * n) || S->blocks.alloc(nb) ||
       S->fac.alloc(nf) || S->ilu.alloc(nb) || S->gst.alloc(n) || S->dbdiag.alloc(n) || S->ones.alloc(ctx->g.plane) || S->idt.alloc(ctx->g.plane) || S->dtv.alloc(n) || S->rhs.alloc(n) || S->x.alloc(n) ||
       S->tmp.alloc(n) || S->jz.alloc(n) || S->qop.alloc(n) || S->seed.alloc(n) || S->hdev.alloc(restart + 2) ||
-      S->ydev.alloc(restart + 2) || S->partial.alloc(multi_dot_partial_size(restart + 2)))
+      S->ydev.alloc(restart + 2) || S->partial.alloc(multi_dot_partial_size(restart + 2)) ||
+      S->hpin.alloc(2 * (size_t)(restart + 2)))
     return sg_fail(ctx, SG_ERR_HIP, "solver allocation failed");
   for (DevBuf* b : {&S->V, &S->Z, &S->blocks, &S->fac, &S->dtv, &S->rhs, &S->x, &S->tmp, &S->jz, &S->qop, &S->seed})
     if (hipMemset(b->p, 0, b->n * sizeof(double)) != hipSuccess) return sg_fail(ctx, SG_ERR_HIP, "memset");
@@ -97,8 +99,9 @@ double host_dot(sg_ctx* ctx, const double* a, const double* b, int* rc) {
   SolverState* S = ctx->solver;
   double out = 0.0;
   hipError_t e = multi_dot(a, 0, 1, b, S->n, S->partial.p, S->hdev.p, false, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(&out, S->hdev.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(S->hpin.p, S->hdev.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) out = S->hpin.p[0];
   *rc = sg_hip(ctx, e, "dot");
   return out;
 }
@@ -291,9 +294,12 @@ int fgmres(sg_ctx* ctx, double rtol, int maxit, int* iters, double* relres) {
       SG_TRY(multi_dot(S->V.p, n, j + 1, W, n, S->partial.p, S->ydev.p, false, ctx->stream, wn1, wn0), "dot");
       SG_TRY(multi_axpy(S->V.p, n, j + 1, S->ydev.p, W, n, ctx->stream, wn1, wn0), "axpy");
       SG_TRY(multi_dot(W, 0, 1, W, n, S->partial.p, wn2, false, ctx->stream, wn1, wn0), "dot");
-      std::vector<double> h1(j + 2), h2(m + 2);
-      SG_TRY(hipMemcpyAsync(h1.data(), S->hdev.p, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
-      SG_TRY(hipMemcpyAsync(h2.data(), S->ydev.p, (m + 2) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+      const double* h1 = S->hpin.p;          // hdev[0..j+1]
+      const double* h2 = S->hpin.p + m + 2;  // ydev[0..m+1]
+      SG_TRY(hipMemcpyAsync(S->hpin.p, S->hdev.p, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+             "D2H");
+      SG_TRY(hipMemcpyAsync(S->hpin.p + m + 2, S->ydev.p, (m + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream), "D2H");
       SG_TRY(hipStreamSynchronize(ctx->stream), "sync");
       const bool reorth = h2[j + 1] < 0.5 * h1[j + 1];  // the device's test on the same doubles
       for (int i = 0; i <= j; ++i) hcol[i] = h1[i] + (reorth ? h2[i] : 0.0);
@@ -326,7 +332,11 @@ int fgmres(sg_ctx* ctx, double rtol, int maxit, int* iters, double* relres) {
       for (int k = i + 1; k <= jend; ++k) s -= H[(size_t)i * m + k] * y[k];
       y[i] = s / H[(size_t)i * m + i];
     }
-    SG_TRY(hipMemcpyAsync(S->ydev.p, y.data(), (jend + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "H2D");
+    // through the pinned buffer: the host next writes it after a later
+    // synchronisation, when this copy has run
+    std::copy(y.begin(), y.begin() + jend + 1, S->hpin.p);
+    SG_TRY(hipMemcpyAsync(S->ydev.p, S->hpin.p, (jend + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+           "H2D");
     SG_TRY(lincomb(S->Z.p, n, jend + 1, S->ydev.p, S->x.p, n, true, ctx->stream), "lincomb");
     if (done) break;
   }
