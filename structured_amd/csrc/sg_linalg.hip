@@ -18,12 +18,13 @@ __device__ __forceinline__ bool gate_open(const double* ga, const double* gb) { 
 
 // out_partial[b*k + i] = sum over slice b of V_i . w.  The grid is
 // DOT_SLICES x (groups of DOT_KC vectors): block (b, g) sums vectors
-// g*DOT_KC .. g*DOT_KC + DOT_KC - 1 over the elements of slice b (the
-// elements b*DOT_BLOCK + t + m*DOT_SLICES*DOT_BLOCK), with its accumulators in
-// registers, then over the block in a fixed order (DPP wavefront sums, then
-// the four waves).  All groups run at once, so a launch keeps every vector's
-// loads in flight together; each group reads w once more, from L2 / MALL
-// after the first.
+// g*DOT_KC .. g*DOT_KC + DOT_KC - 1 over the element pairs of slice b (pairs
+// b*DOT_BLOCK + t + m*DOT_SLICES*DOT_BLOCK, read as 16-byte loads; an odd last
+// element goes to slice 0, thread 0), with its accumulators in registers,
+// then over the block in a fixed order (DPP wavefront sums, then the four
+// waves).  All groups run at once, so a launch keeps every vector's loads in
+// flight together; each group reads w once more, from L2 / MALL after the
+// first.  Vectors start on 16-byte boundaries (the host checks).
 constexpr int DOT_KC = 8;
 constexpr int DOT_SLICES = 512;  // partial sums per dot product
 __global__ __launch_bounds__(DOT_BLOCK) void multi_dot_kernel(const double* V, size_t ldv, int k, const double* w,
@@ -36,12 +37,22 @@ __global__ __launch_bounds__(DOT_BLOCK) void multi_dot_kernel(const double* V, s
   double acc[DOT_KC];
 #pragma unroll
   for (int i = 0; i < DOT_KC; ++i) acc[i] = 0.0;
-#pragma unroll 2
-  for (size_t e = (size_t)blockIdx.x * DOT_BLOCK + threadIdx.x; e < n; e += (size_t)DOT_SLICES * DOT_BLOCK) {
-    const double we = w[e];
+  const size_t n2 = n / 2;
+  const double2* w2 = reinterpret_cast<const double2*>(w);
+  for (size_t e = (size_t)blockIdx.x * DOT_BLOCK + threadIdx.x; e < n2; e += (size_t)DOT_SLICES * DOT_BLOCK) {
+    const double2 we = w2[e];
 #pragma unroll
     for (int i = 0; i < DOT_KC; ++i)
-      if (i < kk) acc[i] += V[(size_t)(i0 + i) * ldv + e] * we;
+      if (i < kk) {
+        const double2 v = reinterpret_cast<const double2*>(V + (size_t)(i0 + i) * ldv)[e];
+        acc[i] += v.x * we.x;
+        acc[i] += v.y * we.y;
+      }
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < DOT_KC; ++i)
+      if (i < kk) acc[i] += V[(size_t)(i0 + i) * ldv + n - 1] * w[n - 1];
   }
 #pragma unroll
   for (int i = 0; i < DOT_KC; ++i) acc[i] = wave_sum_lane63(acc[i]);
@@ -74,8 +85,16 @@ __global__ __launch_bounds__(256) void dot_finalize_kernel(const double* partial
   }
 }
 
+// the kernels read vectors as 16-byte pairs: every vector (V_i = V + i*ldv and w)
+// must start on a 16-byte boundary
+static bool pairs_aligned(const double* V, size_t ldv, int k, const double* w) {
+  const uintptr_t a = (uintptr_t)V | (uintptr_t)w;
+  return (a & 15) == 0 && (k <= 1 || (ldv & 1) == 0);
+}
+
 hipError_t multi_dot(const double* V, size_t ldv, int k, const double* w, size_t n, double* partial, double* out,
                      bool accumulate, hipStream_t s, const double* ga, const double* gb) {
+  if (!pairs_aligned(V, ldv, k, w)) return hipErrorInvalidValue;
   const dim3 grid(DOT_SLICES, (unsigned)((k + DOT_KC - 1) / DOT_KC));
   hipLaunchKernelGGL(multi_dot_kernel, grid, dim3(DOT_BLOCK), 0, s, V, ldv, k, w, n, partial, ga, gb);
   hipLaunchKernelGGL(dot_finalize_kernel, dim3(k), dim3(256), 0, s, partial, DOT_SLICES, k, out, accumulate ? 1 : 0,
@@ -85,54 +104,84 @@ hipError_t multi_dot(const double* V, size_t ldv, int k, const double* w, size_t
 
 size_t multi_dot_partial_size(int kmax) { return (size_t)DOT_SLICES * kmax; }
 
-// w -= sum_i c[i] V_i   (c in device memory).  The vectors are taken eight
-// at a time with their loads issued together ahead of the eight updates (the
-// updates keep the order i = 0, 1, ..., so the result does not depend on the
-// grouping); a loop with one load per update waited for every load in turn.
+// w -= sum_i c[i] V_i   (c in device memory).  Element pairs (16-byte
+// loads), the vectors taken eight at a time with their loads issued together
+// ahead of the eight updates; the updates keep the order i = 0, 1, ..., so
+// every element gets the arithmetic of a plain loop over i.
 constexpr int AXPY_KC = 8;
 __global__ void multi_axpy_kernel(const double* V, size_t ldv, int k, const double* c, double* w, size_t n,
                                   const double* ga, const double* gb) {
   if (!gate_open(ga, gb)) return;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    double s = w[e];
+  const size_t n2 = n / 2;
+  double2* w2 = reinterpret_cast<double2*>(w);
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
+    double2 s = w2[e];
     int i = 0;
     for (; i + AXPY_KC <= k; i += AXPY_KC) {
-      double v[AXPY_KC];
+      double2 v[AXPY_KC];
 #pragma unroll
-      for (int u = 0; u < AXPY_KC; ++u) v[u] = V[(size_t)(i + u) * ldv + e];
+      for (int u = 0; u < AXPY_KC; ++u) v[u] = reinterpret_cast<const double2*>(V + (size_t)(i + u) * ldv)[e];
 #pragma unroll
-      for (int u = 0; u < AXPY_KC; ++u) s -= c[i + u] * v[u];
+      for (int u = 0; u < AXPY_KC; ++u) {
+        s.x -= c[i + u] * v[u].x;
+        s.y -= c[i + u] * v[u].y;
+      }
     }
-    for (; i < k; ++i) s -= c[i] * V[(size_t)i * ldv + e];
-    w[e] = s;
+    for (; i < k; ++i) {
+      const double2 v = reinterpret_cast<const double2*>(V + (size_t)i * ldv)[e];
+      s.x -= c[i] * v.x;
+      s.y -= c[i] * v.y;
+    }
+    w2[e] = s;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    double t = w[n - 1];
+    for (int i = 0; i < k; ++i) t -= c[i] * V[(size_t)i * ldv + n - 1];
+    w[n - 1] = t;
   }
 }
 
 hipError_t multi_axpy(const double* V, size_t ldv, int k, const double* c, double* w, size_t n, hipStream_t s,
                       const double* ga, const double* gb) {
+  if (!pairs_aligned(V, ldv, k, w)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(multi_axpy_kernel, dim3(2048), dim3(256), 0, s, V, ldv, k, c, w, n, ga, gb);
   return hipGetLastError();
 }
 
-// y = sum_i c[i] X_i  (c in device memory), grouped as multi_axpy
+// y = sum_i c[i] X_i  (c in device memory), in pairs and groups as multi_axpy
 __global__ void lincomb_kernel(const double* X, size_t ldx, int k, const double* c, double* y, size_t n, int accumulate) {
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    double s = accumulate ? y[e] : 0.0;
+  const size_t n2 = n / 2;
+  double2* y2 = reinterpret_cast<double2*>(y);
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
+    double2 s = accumulate ? y2[e] : make_double2(0.0, 0.0);
     int i = 0;
     for (; i + AXPY_KC <= k; i += AXPY_KC) {
-      double v[AXPY_KC];
+      double2 v[AXPY_KC];
 #pragma unroll
-      for (int u = 0; u < AXPY_KC; ++u) v[u] = X[(size_t)(i + u) * ldx + e];
+      for (int u = 0; u < AXPY_KC; ++u) v[u] = reinterpret_cast<const double2*>(X + (size_t)(i + u) * ldx)[e];
 #pragma unroll
-      for (int u = 0; u < AXPY_KC; ++u) s += c[i + u] * v[u];
+      for (int u = 0; u < AXPY_KC; ++u) {
+        s.x += c[i + u] * v[u].x;
+        s.y += c[i + u] * v[u].y;
+      }
     }
-    for (; i < k; ++i) s += c[i] * X[(size_t)i * ldx + e];
-    y[e] = s;
+    for (; i < k; ++i) {
+      const double2 v = reinterpret_cast<const double2*>(X + (size_t)i * ldx)[e];
+      s.x += c[i] * v.x;
+      s.y += c[i] * v.y;
+    }
+    y2[e] = s;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    double t = accumulate ? y[n - 1] : 0.0;
+    for (int i = 0; i < k; ++i) t += c[i] * X[(size_t)i * ldx + n - 1];
+    y[n - 1] = t;
   }
 }
 
 hipError_t lincomb(const double* X, size_t ldx, int k, const double* c_dev, double* y, size_t n, bool accumulate,
                    hipStream_t s) {
+  if (!pairs_aligned(X, ldx, k, y)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(lincomb_kernel, dim3(1024), dim3(256), 0, s, X, ldx, k, c_dev, y, n, accumulate ? 1 : 0);
   return hipGetLastError();
 }
