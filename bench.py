@@ -342,7 +342,7 @@ def measured_traffic(ni, nj):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
+            return float(line.split(":", 1)[1].split()[0])
     return None
 
 
