@@ -687,8 +687,12 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
   if (partial) {
     const int lane = tid & 63, wv = tid >> 6;
     const int nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
+    // only the norms the caller asked for (the first nout.count of R^2 then
+    // (V R)^2) are reduced and summed
+    const int nsum = nout.count;
 #pragma unroll
-    for (int k = 0; k < 2 * NV; ++k) sq[k] = wave_sum_lane63(sq[k]);
+    for (int k = 0; k < 2 * NV; ++k)
+      if (k < nsum) sq[k] = wave_sum_lane63(sq[k]);
     if (lane == 63) {
 #pragma unroll
       for (int k = 0; k < 2 * NV; ++k) red[wv][k] = sq[k];
@@ -705,9 +709,12 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
     // a slot never written in this context never passes for parity 0.
     const bool want_neg = fused && (nout.parity & 1u) == 0u;
     __shared__ bool last;
+    // every slot is stored on every fused launch (zero beyond nsum), so that
+    // a slot always holds this launch's parity or the previous launch's
     if (tid < 2 * NV) {
       double s = 0.0;
-      for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
+      if (tid < nsum)
+        for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
       if (want_neg) s = -s;
       __hip_atomic_store(&partial[(size_t)tid * nb + bid], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -731,9 +738,9 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
           const int b = b0 + u * NT + tid;
 #pragma unroll
           for (int k = 0; k < 2 * NV; ++k)
-            v[u][k] = b < nb ? __hip_atomic_load(&partial[(size_t)k * nb + b], __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT)
-                             : (want_neg ? -0.0 : 0.0);
+            v[u][k] = (b < nb && k < nsum) ? __hip_atomic_load(&partial[(size_t)k * nb + b], __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT)
+                                           : (want_neg ? -0.0 : 0.0);
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
