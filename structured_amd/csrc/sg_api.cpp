@@ -430,7 +430,7 @@ int sg_create(int ni, int nj, const double* xv, const double* yv, const sg_frees
   const size_t naos = ctx->ncell() * ctx->nvar, nsoa = ctx->nsoa();
   const size_t npart = residual_partial_count(g) * 2 * ctx->nvar;
   if (ctx->aos_a.alloc(naos) || ctx->aos_b.alloc(naos) || ctx->q.alloc(nsoa) || ctx->r.alloc(nsoa) ||
-      ctx->v.alloc(nsoa) || ctx->w.alloc(nsoa) || ctx->partial.alloc(npart) || ctx->l2.alloc(16) || ctx->counter.alloc(2) ||
+      ctx->v.alloc(nsoa) || ctx->w.alloc(nsoa) || ctx->partial.alloc(npart) || ctx->l2.alloc(16) || ctx->counter.alloc(sg::kNormStateWords) ||
       ctx->wall_cp.alloc(G.ni) || ctx->wall_cf.alloc(G.ni) || ctx->wall_f.alloc(4))
     return fail(SG_ERR_HIP);
   // zero the SoA padding once so padded lanes never hold garbage; the fused

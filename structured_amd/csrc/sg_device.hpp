@@ -5,6 +5,14 @@
 #include <cstddef>
 
 namespace sg {
+// Arrival counters of the fused norm: one per group of residual tiles
+// (kNormGroups, tile index modulo kNormGroups: the XCD the tile runs on) and
+// one for the groups, each on its own 128-byte line; 64-bit words.
+constexpr int kNormGroups = 8, kNormLineWords = 16;
+constexpr int kNormStateWords = (kNormGroups + 1) * kNormLineWords;
+}  // namespace sg
+
+namespace sg {
 
 // Device geometry, struct-of-arrays.  Cell arrays: a[j*ldi + i] (i < nic,
 // j < njc).  Chi-face arrays (faces i = 0..ni-1 between cells i-1 and i):

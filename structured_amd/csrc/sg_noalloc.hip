@@ -18,10 +18,11 @@
 
 namespace {
 
-// Layout: [0, 32) doubles hold the fused norm's arrival counter (one 64-bit
-// word; the rest keeps the partials 256-byte aligned); then the partials,
-// 2 nvar per residual tile, component-major.
-constexpr size_t kCounterDoubles = 32;
+// Layout: [0, 160) doubles hold the fused norm's arrival counters
+// (sg::kNormStateWords 64-bit words; the rest keeps the partials 256-byte
+// aligned); then the partials, 2 nvar per residual tile, component-major.
+constexpr size_t kCounterDoubles = 160;
+static_assert(kCounterDoubles >= (size_t)sg::kNormStateWords && kCounterDoubles % 32 == 0, "counter area");
 constexpr size_t kScratchDoubles = kCounterDoubles + (size_t)SG_NOALLOC_MAX_TILES * 10;
 
 static __device__ __align__(256) double scratch[kScratchDoubles];

@@ -351,8 +351,9 @@ __device__ __forceinline__ SaTerms<S> sa_terms(S r, S n, S ml, const S* gu, cons
 }
 
 // Fused norm output: with l2 set, the block that finishes last finalises the
-// norms.  *state counts the blocks that have finished (0 between launches);
-// parity alternates between consecutive fused launches on the same partials.
+// norms.  state holds the arrival counters (kNormStateWords words: one per
+// tile group, then the count of groups done; all 0 between launches); parity
+// alternates between consecutive fused launches on the same partials.
 struct NormOut {
   unsigned long long* state;
   double* l2;         // ||R_k||_2 (k < NV), then ||V R_k||_2 (NV <= k < 2 NV)
@@ -676,9 +677,15 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
   // operations complete in issue order, so a wait for the count's return
   // issued after the stores would also wait for them to be acknowledged; the
   // parity protocol below makes the order of arrival and stores irrelevant.
+  // Blocks count on the counter of their group (tile index modulo
+  // kNormGroups, one 128-byte line each), so that arrivals on one word come
+  // from one XCD's blocks only; the last of a group counts on the group
+  // counter, and the last group's last block finalises.
+  const int grp = (int)(blockIdx.y * gridDim.x + blockIdx.x) % kNormGroups;
   unsigned long long arrival = 0;
   if (fused && tid == 0)
-    arrival = __hip_atomic_fetch_add(nout.state, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    arrival = __hip_atomic_fetch_add(nout.state + grp * kNormLineWords, 1ull, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
   if (owns) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) out.put(k * g.plane + oidx, res[k]);
@@ -718,14 +725,26 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
       if (want_neg) s = -s;
       __hip_atomic_store(&partial[(size_t)tid * nb + bid], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (fused && tid == 0) last = arrival == (unsigned long long)(nb - 1);
+    if (fused && tid == 0) {
+      // blocks of group grp: grp, grp + G, ... < nb; groups with a block: min(nb, G)
+      const unsigned long long gsize = (unsigned long long)((nb - grp + kNormGroups - 1) / kNormGroups);
+      bool l = false;
+      if (arrival == gsize - 1) {
+        const unsigned long long gdone = __hip_atomic_fetch_add(nout.state + kNormGroups * kNormLineWords, 1ull,
+                                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        l = gdone == (unsigned long long)(min(nb, kNormGroups) - 1);
+      }
+      last = l;
+    }
     if (fused) __syncthreads();
     if (fused && last) {
       // The block that arrives last sums the partials of all blocks in a
       // fixed order (so the norm does not depend on which block it is),
-      // polling each slot until it holds this launch's value; every other
-      // block has arrived, so it is running and stores its partials, and the
-      // wait ends.
+      // polling each slot until it holds this launch's value.  Every other
+      // block has arrived (each group counter reached its group's size before
+      // the group count reached the number of groups: single-location
+      // read-modify-write order), so it is running and stores its partials,
+      // and the wait ends.
       int spins = 0;
       double acc[2 * NV];
 #pragma unroll
@@ -768,7 +787,9 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
         for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
         nout.l2[tid] = sqrt(s);
       }
-      if (tid == 0) __hip_atomic_store(nout.state, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // every arrival of this launch is in: reset the counters
+      if (tid <= kNormGroups)
+        __hip_atomic_store(nout.state + tid * kNormLineWords, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
