@@ -749,28 +749,38 @@ __global__ __launch_bounds__(NT, sizeof(S) == sizeof(double) ? 1024 / NT : 1) vo
       double acc[2 * NV];
 #pragma unroll
       for (int k = 0; k < 2 * NV; ++k) acc[k] = 0.0;
-      // two rows of blocks per round, all loads of a round in flight together
-      for (int b0 = 0; b0 < nb; b0 += 2 * NT) {
-        double v[2][2 * NV];
+      // the plain norms, then (if asked for) the weighted ones; four rows of
+      // blocks per round (one round up to 1,024 blocks), all loads of a round
+      // in flight together
+      constexpr int NR = 4;
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int b = b0 + u * NT + tid;
+      for (int half = 0; half < 2; ++half) {
+        if (half * NV >= nsum) break;
+        for (int b0 = 0; b0 < nb; b0 += NR * NT) {
+          double v[NR][NV];
 #pragma unroll
-          for (int k = 0; k < 2 * NV; ++k)
-            v[u][k] = (b < nb && k < nsum) ? __hip_atomic_load(&partial[(size_t)k * nb + b], __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT)
-                                           : (want_neg ? -0.0 : 0.0);
-        }
+          for (int u = 0; u < NR; ++u) {
+            const int b = b0 + u * NT + tid;
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int b = b0 + u * NT + tid;
-#pragma unroll
-          for (int k = 0; k < 2 * NV; ++k) {
-            while (__builtin_signbit(v[u][k]) != want_neg) {
-              if (++spins > kNormSpinLimit) { v[u][k] = nan; break; }
-              v[u][k] = __hip_atomic_load(&partial[(size_t)k * nb + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int kk = 0; kk < NV; ++kk) {
+              const int k = half * NV + kk;
+              v[u][kk] = (b < nb && k < nsum) ? __hip_atomic_load(&partial[(size_t)k * nb + b], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT)
+                                              : (want_neg ? -0.0 : 0.0);
             }
-            acc[k] += fabs(v[u][k]);
+          }
+#pragma unroll
+          for (int u = 0; u < NR; ++u) {
+            const int b = b0 + u * NT + tid;
+#pragma unroll
+            for (int kk = 0; kk < NV; ++kk) {
+              const int k = half * NV + kk;
+              while (__builtin_signbit(v[u][kk]) != want_neg) {
+                if (++spins > kNormSpinLimit) { v[u][kk] = nan; break; }
+                v[u][kk] = __hip_atomic_load(&partial[(size_t)k * nb + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              }
+              acc[k] += fabs(v[u][kk]);
+            }
           }
         }
       }
