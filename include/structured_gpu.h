@@ -134,7 +134,7 @@ typedef struct sg_adjoint_opts {
                       6 chunked line ILU(0) on the 13-point stencil, 7 the
                       same on the 3x3-box pattern                           */
   int lhs_order;   /* 1: the preconditioner factors the first-order Jacobian */
-  int pc_width;    /* chunk width of precond 6 / 7 (8, 16 or 32); 0 keeps the
+  int pc_width;    /* chunk width of precond 6 / 7 (8, 16, 24 or 32); 0 keeps the
                       context's current width                               */
   double pc_cfl;   /* the preconditioner factors I/dt - J at this CFL        */
 } sg_adjoint_opts;
@@ -230,7 +230,7 @@ int sg_ls_set_lhs(sg_ctx* ctx, const double* q_aos, const double* dt_aos);
 /* Preconditioner settings for the following set_lhs (after preallocate):
  * lhs_order as in sg_newton_opts, the factor (>= 1) by which the
  * preconditioner's 1/dt is larger than the operator's, and the chunk width of
- * precond 6 (8, 16 or 32; 0 keeps the current one). */
+ * precond 6 / 7 (8, 16, 24 or 32; 0 keeps the current one). */
 int sg_ls_set_pc(sg_ctx* ctx, int lhs_order, double pc_dt_factor, int pc_width);
 int sg_ls_set_rhs(sg_ctx* ctx, const double* rhs_aos);
 int sg_ls_solve(sg_ctx* ctx, double* dq_aos, int* iters, double* relres);
@@ -290,7 +290,7 @@ typedef struct sg_newton_opts {
    * the preconditioner factors the Jacobian of the first-order residual
    * (src/solver/solver.cpp:92-101); the Krylov operator stays exact. */
   int lhs_order;
-  int pc_width;  /* chunk width of precond 6: 8, 16 or 32 (0: 16) */
+  int pc_width;  /* chunk width of precond 6 / 7: 8, 16, 24 or 32 (0: 16) */
 } sg_newton_opts;
 
 typedef struct sg_solve_report {

@@ -531,7 +531,7 @@ static void cilu_apply_t(int nic, int njc, int ldi, double* store, const double*
                        tcopy, b, z, prog, epoch);
 }
 
-bool cilu_width_ok(int w) { return w == 8 || w == 16 || w == 32; }
+bool cilu_width_ok(int w) { return w == 8 || w == 16 || w == 24 || w == 32; }
 
 template <bool NINE>
 static void cilu_factor_n(int nic, int njc, int ldi, int nv, int w, const double* idt, const double* blocks,
@@ -539,10 +539,12 @@ static void cilu_factor_n(int nic, int njc, int ldi, int nv, int w, const double
   if (nv == 5) {
     if (w == 8) cilu_factor_t<5, 8, NINE>(nic, njc, ldi, idt, blocks, store, s);
     else if (w == 16) cilu_factor_t<5, 16, NINE>(nic, njc, ldi, idt, blocks, store, s);
+    else if (w == 24) cilu_factor_t<5, 24, NINE>(nic, njc, ldi, idt, blocks, store, s);
     else cilu_factor_t<5, 32, NINE>(nic, njc, ldi, idt, blocks, store, s);
   } else {
     if (w == 8) cilu_factor_t<4, 8, NINE>(nic, njc, ldi, idt, blocks, store, s);
     else if (w == 16) cilu_factor_t<4, 16, NINE>(nic, njc, ldi, idt, blocks, store, s);
+    else if (w == 24) cilu_factor_t<4, 24, NINE>(nic, njc, ldi, idt, blocks, store, s);
     else cilu_factor_t<4, 32, NINE>(nic, njc, ldi, idt, blocks, store, s);
   }
 }
@@ -553,10 +555,12 @@ static void cilu_apply_n(int nic, int njc, int ldi, int nv, int w, double* store
   if (nv == 5) {
     if (w == 8) cilu_apply_t<5, 8, NINE>(nic, njc, ldi, store, b, z, transpose, s);
     else if (w == 16) cilu_apply_t<5, 16, NINE>(nic, njc, ldi, store, b, z, transpose, s);
+    else if (w == 24) cilu_apply_t<5, 24, NINE>(nic, njc, ldi, store, b, z, transpose, s);
     else cilu_apply_t<5, 32, NINE>(nic, njc, ldi, store, b, z, transpose, s);
   } else {
     if (w == 8) cilu_apply_t<4, 8, NINE>(nic, njc, ldi, store, b, z, transpose, s);
     else if (w == 16) cilu_apply_t<4, 16, NINE>(nic, njc, ldi, store, b, z, transpose, s);
+    else if (w == 24) cilu_apply_t<4, 24, NINE>(nic, njc, ldi, store, b, z, transpose, s);
     else cilu_apply_t<4, 32, NINE>(nic, njc, ldi, store, b, z, transpose, s);
   }
 }

@@ -384,7 +384,7 @@ int sg_ls_preallocate(sg_ctx* ctx, int restart, int precond, int maxit, double r
 
 int sg_ls_set_pc(sg_ctx* ctx, int lhs_order, double pc_dt_factor, int pc_width) {
   if (!ctx || !ctx->solver || lhs_order < 0 || lhs_order > 2 || !(pc_dt_factor >= 1.0)) return SG_ERR_ARG;
-  if (pc_width != 0 && !cilu_width_ok(pc_width)) return sg_fail(ctx, SG_ERR_ARG, "pc_width must be 8, 16 or 32");
+  if (pc_width != 0 && !cilu_width_ok(pc_width)) return sg_fail(ctx, SG_ERR_ARG, "pc_width must be 8, 16, 24 or 32");
   ctx->solver->lhs_order = lhs_order;
   ctx->solver->pc_boost = pc_dt_factor;
   if (pc_width) ctx->solver->pc_width = pc_width;
@@ -698,7 +698,7 @@ int sg_adjoint_solve(sg_ctx* ctx, const double* q_aos, const double* rhs_aos, do
     return sg_fail(ctx, SG_ERR_ARG, "adjoint precond must be 0, 4, 5, 6 or 7");
   if (o->lhs_order < 0 || o->lhs_order > 2) return sg_fail(ctx, SG_ERR_ARG, "adjoint lhs_order must be 0, 1 or 2");
   if (o->pc_width != 0 && !cilu_width_ok(o->pc_width))
-    return sg_fail(ctx, SG_ERR_ARG, "adjoint pc_width must be 0, 8, 16 or 32");
+    return sg_fail(ctx, SG_ERR_ARG, "adjoint pc_width must be 0, 8, 16, 24 or 32");
   int rc;
   if ((rc = sg_ls_preallocate(ctx, o->restart, o->precond, o->maxit, o->rtol))) return rc;
   SolverState* S = ctx->solver;

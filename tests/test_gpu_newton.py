@@ -113,7 +113,7 @@ def test_ilu_matches_numpy_model_sa(precond):
     assert err <= 1e-9
 
 
-@pytest.mark.parametrize("width,sa", [(8, False), (16, False), (8, True), (32, True)])
+@pytest.mark.parametrize("width,sa", [(8, False), (16, False), (8, True), (24, True), (32, True)])
 @pytest.mark.parametrize("transpose", [False, True])
 @pytest.mark.parametrize("precond", [6, 7])
 def test_chunked_ilu_matches_numpy_model(width, sa, transpose, precond):
@@ -125,7 +125,7 @@ def test_chunked_ilu_matches_numpy_model(width, sa, transpose, precond):
     box pattern drops fill only)."""
     import copy
     import precond_model as pm
-    ni = 41 if width == 32 else 13
+    ni = 41 if width >= 24 else 13
     case = cases.sa_bump(ni, 9, dy1=1e-3) if sa else cases.laminar_plate(ni, 9, dy1=1e-2)
     q = case.perturbed_state(seed=6, amp=0.01)
     eq = lib.Equation(case)
