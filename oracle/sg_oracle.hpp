@@ -24,9 +24,17 @@
 //       (tests/oracle_ffi.py explicit_solve) are pinned against the
 //       reference's own Solver::solve.
 //   Spalart-Allmaras terms, beta field, wall objective, MLP:
-//       the reference has none of these.  Pinned by the published SA-noft2
-//       definition (closed-form checks), central finite differences and
-//       invariants only: "parity unpinned" against the reference.
+//       the reference has none of these, so no reference output exists to
+//       compare with.  Each is pinned by value instead: the published
+//       SA-noft2 definition in closed form (production/destruction, the
+//       diffusion, c_b2 and density-gradient terms on linear profiles, the
+//       eddy viscosity and conductivity in the viscous fluxes, the ghost
+//       values per boundary type: tests/test_oracle_sa.py and
+//       tests/test_oracle_pins.py), the wall distance (y_c on the flat plate,
+//       brute-force point-to-segment distance on the bump), the closure
+//       features on a uniform shear, the MLP against a numpy model, and the
+//       objective F against its definition (tests/test_gpu_contract.py);
+//       central finite differences check the tangents.
 #pragma once
 #include <algorithm>
 #include <cmath>

@@ -1,8 +1,10 @@
 """Pins for the oracle's Spalart-Allmaras terms, which have no counterpart in
-the reference ("parity unpinned" against it): the SA-noft2 source against a
-closed-form evaluation of the published model (Spalart & Allmaras 1994 /
-NASA TMR, SA-neg branch for nu~ < 0), and the dual-number J.v against
-central differences of the oracle residual."""
+the reference (so no reference output to compare with; they are pinned by
+value instead): the SA-noft2 source against a closed-form evaluation of the
+published model (Spalart & Allmaras 1994 / NASA TMR, SA-neg branch for
+nu~ < 0), and the dual-number J.v against central differences of the oracle
+residual.  The diffusion, c_b2, density-gradient, eddy-viscosity and ghost
+terms are pinned in closed form in test_oracle_pins.py."""
 import numpy as np
 import pytest
 
