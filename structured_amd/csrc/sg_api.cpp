@@ -207,8 +207,10 @@ bool BcCompiled::compile(const HostGeom& g, const sg_bc* bcs, int nbc, bool visc
   // interior cell that reads a ghost (those within one cell of it) must see
   // the ghost's interior operands inside its 13-point stencil.
   local = true;
-  auto in_stencil = [](int di, int dj) {
-    return (std::abs(di) <= 1 && std::abs(dj) <= 1) || (di == 0 && std::abs(dj) == 2) || (dj == 0 && std::abs(di) == 2);
+  local_box = true;
+  auto in_box = [](int di, int dj) { return std::abs(di) <= 1 && std::abs(dj) <= 1; };
+  auto in_stencil = [&](int di, int dj) {
+    return in_box(di, dj) || (di == 0 && std::abs(dj) == 2) || (dj == 0 && std::abs(di) == 2);
   };
   for (int gi = 0; gi <= nic + 1 && local; ++gi)
     for (int gj = 0; gj <= njc + 1 && local; ++gj) {
@@ -228,8 +230,11 @@ bool BcCompiled::compile(const HostGeom& g, const sg_bc* bcs, int nbc, bool visc
       for (int ui = gi - 1; ui <= gi + 1 && local; ++ui)
         for (int uj = gj - 1; uj <= gj + 1 && local; ++uj) {
           if (ui < 1 || ui > nic || uj < 1 || uj > njc) continue;  // users are interior cells
-          for (int r : leaves)
-            if (!in_stencil(r % nic + 1 - ui, r / nic + 1 - uj)) { local = false; break; }
+          for (int r : leaves) {
+            const int di = r % nic + 1 - ui, dj = r / nic + 1 - uj;
+            if (!in_box(di, dj)) local_box = false;
+            if (!in_stencil(di, dj)) { local = false; local_box = false; break; }
+          }
         }
     }
   // Edge ghosts feed the boundary faces; corners feed the viscous stencil.

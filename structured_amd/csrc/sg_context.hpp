@@ -79,6 +79,9 @@ struct BcCompiled {
   // tangentially and two normally (so every Jacobian coupling stays inside
   // the 13-point stencil); false for wake / wide periodic boundaries
   bool local = true;
+  // the same inside the 3x3 box (so the first-order Jacobian, whose interior
+  // couplings are the box, stays inside the box)
+  bool local_box = true;
   bool compile(const HostGeom& g, const sg_bc* bcs, int nbc, bool viscous);
 };
 

@@ -5,6 +5,8 @@
 // i-column).  All vectors are SoA with zero padding (length nvar*plane).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "sg_device.hpp"
 #include "sg_linalg.hpp"
 
@@ -207,16 +209,19 @@ hipError_t shift_minus(const double* z, const double* idt, const double* jz, dou
   return hipGetLastError();
 }
 
-// v = e_m on the cells of colour (a, b) = (i mod 5, j mod 5), 0 elsewhere.
-__global__ void seed_kernel(int nic, int njc, int ldi, int nv, int a, int b, int m, double* v) {
+// v = e_m on the cells of colour (a, b) = (i mod per, j mod per), 0
+// elsewhere.  per = 5 separates the 13-point stencil (reach 2), per = 3 the
+// 3x3 box (reach 1).
+__global__ void seed_kernel(int nic, int njc, int ldi, int nv, int per, int a, int b, int m, double* v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= nic) return;
   const size_t plane = (size_t)njc * ldi, c = (size_t)j * ldi + i;
-  const bool on = (i % 5 == a) && (j % 5 == b);
+  const bool on = (i % per == a) && (j % per == b);
   for (int k = 0; k < nv; ++k) v[k * plane + c] = (on && k == m) ? 1.0 : 0.0;
 }
-hipError_t seed_colour(int nic, int njc, int ldi, int nv, int a, int b, int m, double* v, hipStream_t s) {
-  hipLaunchKernelGGL(seed_kernel, dim3((nic + 127) / 128, njc), dim3(128), 0, s, nic, njc, ldi, nv, a, b, m, v);
+hipError_t seed_colour(int nic, int njc, int ldi, int nv, int per, int a, int b, int m, double* v, hipStream_t s) {
+  if (per != 3 && per != 5) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(seed_kernel, dim3((nic + 127) / 128, njc), dim3(128), 0, s, nic, njc, ldi, nv, per, a, b, m, v);
   return hipGetLastError();
 }
 
@@ -232,23 +237,39 @@ __device__ __forceinline__ int slot_of(int di, int dj) {
 }
 
 // From J.v of colour (a,b), variable m: row cell r holds column m of the
-// block J[r, c] for the unique colour-(a,b) cell c in r's 5x5 window.
+// block J[r, c] for the unique colour-(a,b) cell c in r's per x per window.
 // Blocks are struct-of-arrays: blocks[((slot*nv + row)*nv + col)*plane + cell].
-__global__ void extract_kernel(int nic, int njc, int ldi, int nv, int a, int b, int m, const double* jv, double* blocks) {
+__global__ void extract_kernel(int nic, int njc, int ldi, int nv, int per, int a, int b, int m, const double* jv,
+                               double* blocks) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= nic) return;
-  int di = ((a - i) % 5 + 5) % 5;
-  if (di > 2) di -= 5;
-  int dj = ((b - j) % 5 + 5) % 5;
-  if (dj > 2) dj -= 5;
+  const int h = per / 2;
+  int di = ((a - i) % per + per) % per;
+  if (di > h) di -= per;
+  int dj = ((b - j) % per + per) % per;
+  if (dj > h) dj -= per;
   const int slot = slot_of(di, dj);
   if (slot < 0) return;
   const size_t plane = (size_t)njc * ldi, c = (size_t)j * ldi + i;
   for (int k = 0; k < nv; ++k) blocks[((size_t)(slot * nv + k) * nv + m) * plane + c] = jv[k * plane + c];
 }
-hipError_t extract_blocks(int nic, int njc, int ldi, int nv, int a, int b, int m, const double* jv, double* blocks,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(extract_kernel, dim3((nic + 127) / 128, njc), dim3(128), 0, s, nic, njc, ldi, nv, a, b, m, jv, blocks);
+hipError_t extract_blocks(int nic, int njc, int ldi, int nv, int per, int a, int b, int m, const double* jv,
+                          double* blocks, hipStream_t s) {
+  if (per != 3 && per != 5) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(extract_kernel, dim3((nic + 127) / 128, njc), dim3(128), 0, s, nic, njc, ldi, nv, per, a, b, m,
+                     jv, blocks);
+  return hipGetLastError();
+}
+
+// Zeroes n doubles (a kernel rather than a memset, so that the call may be
+// captured into a graph).
+__global__ void zero_kernel(double* x, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    x[e] = 0.0;
+}
+hipError_t zero_fill(double* x, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, s, x, n);
   return hipGetLastError();
 }
 

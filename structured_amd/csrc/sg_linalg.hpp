@@ -24,9 +24,12 @@ hipError_t axpby(double a, const double* x, double b, double* y, size_t n, hipSt
 // w = z * idt (per cell) - jz
 hipError_t shift_minus(const double* z, const double* idt, const double* jz, double* w, size_t plane, int nv,
                        hipStream_t s);
-hipError_t seed_colour(int nic, int njc, int ldi, int nv, int a, int b, int m, double* v, hipStream_t s);
-hipError_t extract_blocks(int nic, int njc, int ldi, int nv, int a, int b, int m, const double* jv, double* blocks,
-                          hipStream_t s);
+// Coloured Jacobian extraction with colour period per (5: the 13-point
+// stencil; 3: the 3x3 box)
+hipError_t seed_colour(int nic, int njc, int ldi, int nv, int per, int a, int b, int m, double* v, hipStream_t s);
+hipError_t extract_blocks(int nic, int njc, int ldi, int nv, int per, int a, int b, int m, const double* jv,
+                          double* blocks, hipStream_t s);
+hipError_t zero_fill(double* x, size_t n, hipStream_t s);
 hipError_t line_factor(int nic, int njc, int ldi, int nv, const double* idt, const double* blocks, double* fac,
                        bool jacobi, hipStream_t s);
 hipError_t line_solve(int nic, int njc, int ldi, int nv, const double* fac, const double* b, double* x, hipStream_t s);

@@ -158,22 +158,32 @@ int apply_operator(sg_ctx* ctx, const double* z, double* w) {
   return SG_OK;
 }
 
-// Blocks of J at qop by 5 x 5 colouring of J.v.  order > 0 evaluates the
+// Blocks of J at qop by coloured J.v products.  order > 0 evaluates the
 // residual at that MUSCL order (the reference's lhs_order: a first-order
-// Jacobian for the implicit operator, solver.cpp:92-101).
+// Jacobian for the implicit operator, solver.cpp:92-101).  The exact
+// Jacobian couples the 13-point stencil and takes a 5 x 5 colouring (125
+// products for nv = 5).  The first-order one couples the 3x3 box only, so
+// when every ghost cell also depends on interior cells inside the box of the
+// cells that read it, a 3 x 3 colouring separates the columns (45 products)
+// and the +-2 slots are zero.
 int extract_jacobian(sg_ctx* ctx, int order = 0, double* dst = nullptr) {
   sg::DevPhys ph = ctx->ph;
   if (order > 0) ph.order = order;
   if (!dst) dst = ctx->solver->blocks.p;
   SolverState* S = ctx->solver;
   const int nic = ctx->geo.nic, njc = ctx->geo.njc, ldi = ctx->g.ldi, nv = ctx->nvar;
-  for (int a = 0; a < 5; ++a)
-    for (int b = 0; b < 5; ++b)
+  const int per = (ph.order == 1 && ctx->bcc.local_box) ? 3 : 5;
+  if (per == 3) {  // slots 9-12 hold the (0,+-2) and (+-2,0) blocks
+    const size_t slot = (size_t)nv * nv * ctx->g.plane;
+    SG_TRY(zero_fill(dst + 9 * slot, 4 * slot, ctx->stream), "zero");
+  }
+  for (int a = 0; a < per; ++a)
+    for (int b = 0; b < per; ++b)
       for (int m = 0; m < nv; ++m) {
-        SG_TRY(seed_colour(nic, njc, ldi, nv, a, b, m, S->seed.p, ctx->stream), "seed");
+        SG_TRY(seed_colour(nic, njc, ldi, nv, per, a, b, m, S->seed.p, ctx->stream), "seed");
         SG_TRY(launch_jv(ctx->g, ph, ctx->bc, ctx->sa, S->qop.p, S->seed.p, nullptr, S->jz.p, ctx->stream, nullptr,
                          ctx->clo), "jv");
-        SG_TRY(extract_blocks(nic, njc, ldi, nv, a, b, m, S->jz.p, dst, ctx->stream), "extract");
+        SG_TRY(extract_blocks(nic, njc, ldi, nv, per, a, b, m, S->jz.p, dst, ctx->stream), "extract");
       }
   return SG_OK;
 }
