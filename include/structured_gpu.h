@@ -411,6 +411,9 @@ int sg_soa_to_aos_dev(sg_ctx* ctx, const double* soa_dev, double* aos_dev, int n
 /* coef: 7 doubles for SG_CLOSURE_ANALYTIC; mlp_params: SG_MLP_NPARAM floats
  * (host) for SG_CLOSURE_MLP; unused arguments may be NULL. */
 int sg_set_closure(sg_ctx* ctx, int kind, const double* coef, const float* mlp_params);
+/* The same as sg_set_closure(ctx, SG_CLOSURE_MLP, NULL, params): loads the
+ * network's SG_MLP_NPARAM weights (host floats) and selects the MLP closure. */
+int sg_set_mlp_weights(sg_ctx* ctx, const float* params);
 /* Features [nic][njc][5] and the beta the closure gives [nic][njc] at q
  * (either may be NULL).  With SG_CLOSURE_FIELD, beta is the field. */
 int sg_closure_eval(sg_ctx* ctx, const double* q_aos, double* features, double* beta);

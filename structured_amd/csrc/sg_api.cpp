@@ -501,6 +501,11 @@ int sg_set_closure(sg_ctx* ctx, int kind, const double* coef, const float* mlp_p
   return SG_OK;
 }
 
+int sg_set_mlp_weights(sg_ctx* ctx, const float* params) {
+  if (!ctx || !params) return SG_ERR_ARG;
+  return sg_set_closure(ctx, SG_CLOSURE_MLP, nullptr, params);
+}
+
 int sg_closure_eval(sg_ctx* ctx, const double* q_aos, double* features, double* beta) {
   if (!ctx || !q_aos) return SG_ERR_ARG;
   if (!ctx->sa) return sg_fail(ctx, SG_ERR_ARG, "closure features need the SA model");

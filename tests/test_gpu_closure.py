@@ -98,6 +98,24 @@ def test_mlp_closure_beta_vs_oracle_and_network(k):
     assert np.array_equal(b, b2)
 
 
+def test_set_mlp_weights_equals_set_closure():
+    """sg_set_mlp_weights is sg_set_closure with the MLP kind: the same
+    residual bit for bit; a NULL weight vector is an argument error."""
+    import ctypes as C
+    case = sa_case(0)
+    q = case.perturbed_state(seed=6)
+    p = mlp_params(seed=8)
+    eq = lib.Equation(case)
+    eq.set_closure(2, params=p)
+    r1, _ = eq.calc_residual(q)
+    eq.set_closure(0)
+    so = lib.load()
+    assert so.sg_set_mlp_weights(eq.h, C.c_void_p(p.ctypes.data)) == 0
+    r2, _ = eq.calc_residual(q)
+    assert np.array_equal(r1, r2)
+    assert so.sg_set_mlp_weights(eq.h, None) != 0
+
+
 def test_mlp_closure_residual_is_field_residual_at_its_beta():
     case = sa_case(0)
     q = case.perturbed_state(seed=6)
