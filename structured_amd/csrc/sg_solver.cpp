@@ -5,18 +5,22 @@
 // src/linearsolver/ls_eigen.h:25-31) but never forms the matrix: set_lhs
 // stores the linearisation state q and the local time steps, and the
 // operator is A z = z/dt - J(q) z with J z from the tangent kernel.  The
-// preconditioner is built from the exact Jacobian blocks J[r, r] and
-// J[r, r -/+ jhat], extracted by a 5x5 cell colouring of J.v products, and
-// factored per i-column as a block-tridiagonal j-line system (or its block
-// diagonal).  The Krylov solver is right-preconditioned restarted FGMRES with
-// classical Gram-Schmidt and one re-orthogonalisation (CGS2).
+// preconditioners are built from Jacobian blocks extracted by coloured J.v
+// products (a 5 x 5 colouring for the exact 13-point Jacobian, 3 x 3 for the
+// first-order one of lhs_order 1 when its couplings stay in the 3x3 box) and
+// factored as block Jacobi, j-line, line Gauss-Seidel, multicolour or
+// natural-order block ILU(0), or the chunked line ILU(0) of sg_cilu.hip.
+// The Krylov solver is right-preconditioned restarted FGMRES with classical
+// Gram-Schmidt and a second pass only where the DGKS test asks for it,
+// decided on the device.
 //
 // sg_solve runs the pseudo-transient Newton loop of Solver::step/solve
 // (src/solver/solver.cpp:52-236): dt from the CFL, residual and norms,
 // Jacobian, linear solve, q += UR dq, then the CFL and under-relaxation
-// ramps.  Unlike the reference, whose tolerance test is disabled (`&& 0`,
-// solver.cpp:145), the loop stops when every equation's L2 norm is below
-// the tolerance.
+// ramps, with the reference's iteration_max + 1 linear solves.  Unlike the
+// reference, whose tolerance test is disabled (`&& 0`, solver.cpp:145), the
+// loop also stops when every equation's norm is below the tolerance, or on a
+// stall near it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -518,11 +522,10 @@ int sg_solve(sg_ctx* ctx, double* q_aos, const sg_newton_opts* o, sg_solve_repor
     if (it > o->iteration_max) break;
     lmax_hist.push_back(lmax);
     if (o->stall_window > 0 && (int)lmax_hist.size() > 2 * o->stall_window) {
-      const size_t w = (size_t)o->stall_window, m = lmax_hist.size();
+      const size_t w = (size_t)o->stall_window;
       const double before = *std::min_element(lmax_hist.begin(), lmax_hist.end() - w);
       const double recent = *std::min_element(lmax_hist.end() - w, lmax_hist.end());
       if (recent < 1e3 * o->tolerance && recent > 0.5 * before) { stalled = true; break; }
-      (void)m;
     }
     Timer tj;
     S->pc_boost = (o->pc_cfl_max > 0.0 && cfl > o->pc_cfl_max) ? cfl / o->pc_cfl_max : 1.0;
