@@ -1,8 +1,11 @@
 // Device linear algebra for the matrix-free Newton-Krylov solve:
-// deterministic multi-dot products (CGS2 Gram-Schmidt), multi-axpy, scaling,
-// colour seeding and extraction of Jacobian blocks from J.v products, and the
-// j-line block-tridiagonal preconditioner (factor + solve, one thread per
-// i-column).  All vectors are SoA with zero padding (length nvar*plane).
+// deterministic multi-dot products and multi-axpys for the Gram-Schmidt
+// passes (with device-side gates for the selective second pass), scaling,
+// colour seeding and extraction of Jacobian blocks from J.v products, the
+// block preconditioners other than the chunked line ILU (j-line
+// block-tridiagonal, line Gauss-Seidel, multicolour and natural-order block
+// ILU(0)), and the adjoint products J^T psi and (dR/dbeta)^T psi from the
+// blocks.  All vectors are SoA with zero padding (length nvar*plane).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
