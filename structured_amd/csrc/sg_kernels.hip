@@ -1,6 +1,6 @@
-// Kernel launchers of the residual / tangent path and the small helper
-// kernels (norm finalisation, local time step, layout conversion, wall
-// quantities).  Everything is FP64.
+// Kernel launchers of the residual / tangent path (the residual kernel
+// finalises its own norms) and the small helper kernels (local time step,
+// layout conversion, wall quantities).  Everything is FP64.
 #include <hip/hip_runtime.h>
 
 #include "sg_kernels.hpp"
