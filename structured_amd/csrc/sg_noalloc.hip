@@ -2,7 +2,7 @@
 // hipMalloc (code under HIP graph capture, tight serving loops).
 //
 // The fused residual's scratch -- the per-block partial sums of the L2 norms
-// and the arrival counter of the last-block finalisation -- lives in a
+// and the arrival counters of the last-block finalisation -- lives in a
 // file-scope device array instead of context buffers.  Each GPU that the
 // library runs on holds its own copy of the array until the process ends; the
 // context looks its address up once per device at creation
