@@ -18,8 +18,10 @@ namespace sg {
 constexpr int DOT_BLOCK = 256;
 
 // Gate of the conditional kernels: with ga and gb (device) the kernel runs
-// only when *ga < *gb / 2, else it returns at once (nothing written).
-__device__ __forceinline__ bool gate_open(const double* ga, const double* gb) { return !ga || *ga < 0.5 * *gb; }
+// only when *ga < gf * *gb, else it returns at once (nothing written).
+__device__ __forceinline__ bool gate_open(const double* ga, const double* gb, double gf) {
+  return !ga || *ga < gf * *gb;
+}
 
 // out_partial[b*k + i] = sum over slice b of V_i . w.  The grid is
 // DOT_SLICES x (groups of DOT_KC vectors): block (b, g) sums vectors
@@ -34,8 +36,8 @@ constexpr int DOT_KC = 8;
 constexpr int DOT_SLICES = 512;  // partial sums per dot product
 __global__ __launch_bounds__(DOT_BLOCK) void multi_dot_kernel(const double* V, size_t ldv, int k, const double* w,
                                                               size_t n, double* partial, const double* ga,
-                                                              const double* gb) {
-  if (!gate_open(ga, gb)) return;
+                                                              const double* gb, double gf) {
+  if (!gate_open(ga, gb, gf)) return;
   __shared__ double red[DOT_BLOCK / 64][DOT_KC];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i0 = (int)blockIdx.y * DOT_KC, kk = min(DOT_KC, k - i0);
@@ -75,8 +77,9 @@ __global__ __launch_bounds__(DOT_BLOCK) void multi_dot_kernel(const double* V, s
 
 // out[i] = sum_b partial[b*k + i]  (+ optional add into out when accumulate)
 __global__ __launch_bounds__(256) void dot_finalize_kernel(const double* partial, int nb, int k, double* out,
-                                                           int accumulate, const double* ga, const double* gb) {
-  if (!gate_open(ga, gb)) return;
+                                                           int accumulate, const double* ga, const double* gb,
+                                                           double gf) {
+  if (!gate_open(ga, gb, gf)) return;
   const int i = blockIdx.x;
   __shared__ double red[4];
   double s = 0.0;
@@ -98,12 +101,12 @@ static bool pairs_aligned(const double* V, size_t ldv, int k, const double* w) {
 }
 
 hipError_t multi_dot(const double* V, size_t ldv, int k, const double* w, size_t n, double* partial, double* out,
-                     bool accumulate, hipStream_t s, const double* ga, const double* gb) {
+                     bool accumulate, hipStream_t s, const double* ga, const double* gb, double gf) {
   if (!pairs_aligned(V, ldv, k, w)) return hipErrorInvalidValue;
   const dim3 grid(DOT_SLICES, (unsigned)((k + DOT_KC - 1) / DOT_KC));
-  hipLaunchKernelGGL(multi_dot_kernel, grid, dim3(DOT_BLOCK), 0, s, V, ldv, k, w, n, partial, ga, gb);
+  hipLaunchKernelGGL(multi_dot_kernel, grid, dim3(DOT_BLOCK), 0, s, V, ldv, k, w, n, partial, ga, gb, gf);
   hipLaunchKernelGGL(dot_finalize_kernel, dim3(k), dim3(256), 0, s, partial, DOT_SLICES, k, out, accumulate ? 1 : 0,
-                     ga, gb);
+                     ga, gb, gf);
   return hipGetLastError();
 }
 
@@ -115,8 +118,8 @@ size_t multi_dot_partial_size(int kmax) { return (size_t)DOT_SLICES * kmax; }
 // every element gets the arithmetic of a plain loop over i.
 constexpr int AXPY_KC = 8;
 __global__ void multi_axpy_kernel(const double* V, size_t ldv, int k, const double* c, double* w, size_t n,
-                                  const double* ga, const double* gb) {
-  if (!gate_open(ga, gb)) return;
+                                  const double* ga, const double* gb, double gf) {
+  if (!gate_open(ga, gb, gf)) return;
   const size_t n2 = n / 2;
   double2* w2 = reinterpret_cast<double2*>(w);
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
@@ -147,9 +150,9 @@ __global__ void multi_axpy_kernel(const double* V, size_t ldv, int k, const doub
 }
 
 hipError_t multi_axpy(const double* V, size_t ldv, int k, const double* c, double* w, size_t n, hipStream_t s,
-                      const double* ga, const double* gb) {
+                      const double* ga, const double* gb, double gf) {
   if (!pairs_aligned(V, ldv, k, w)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(multi_axpy_kernel, dim3(2048), dim3(256), 0, s, V, ldv, k, c, w, n, ga, gb);
+  hipLaunchKernelGGL(multi_axpy_kernel, dim3(2048), dim3(256), 0, s, V, ldv, k, c, w, n, ga, gb, gf);
   return hipGetLastError();
 }
 

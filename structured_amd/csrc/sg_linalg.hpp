@@ -10,13 +10,14 @@ constexpr int NSLOT = 13;  // Jacobian stencil blocks kept per cell
 
 size_t multi_dot_partial_size(int kmax);
 // out[i] = V_i . w for i < k (V_i = V + i*ldv); out is device memory.  With
-// ga and gb (device doubles) the kernels run only if *ga < *gb / 2 and
+// ga and gb (device doubles) the kernels run only if *ga < gf * *gb and
 // otherwise write nothing: a conditional pass decided on the device.
 hipError_t multi_dot(const double* V, size_t ldv, int k, const double* w, size_t n, double* partial, double* out,
-                     bool accumulate, hipStream_t s, const double* ga = nullptr, const double* gb = nullptr);
+                     bool accumulate, hipStream_t s, const double* ga = nullptr, const double* gb = nullptr,
+                     double gf = 0.5);
 // w -= sum_i c[i] V_i, c device memory; gated as multi_dot
 hipError_t multi_axpy(const double* V, size_t ldv, int k, const double* c, double* w, size_t n, hipStream_t s,
-                      const double* ga = nullptr, const double* gb = nullptr);
+                      const double* ga = nullptr, const double* gb = nullptr, double gf = 0.5);
 // y (+)= sum_i c[i] X_i, c device memory
 hipError_t lincomb(const double* X, size_t ldx, int k, const double* c_dev, double* y, size_t n, bool accumulate,
                    hipStream_t s);
